@@ -1,0 +1,157 @@
+/* xted: MI355X-native parallel Zhang-Shasha tree edit distance, C ABI.
+ *
+ * This is the thin C layer the C++ drop-in (include/TED_C++.h) and any other
+ * host language bind to. Plain pointers and sizes only; no exceptions cross
+ * it; every entry point returns an xted status code and the thread's last
+ * error message is available from xted_last_error().
+ *
+ * Tree encoding: n nodes in preorder, parent[0] = -1, 0 <= parent[i] < i and
+ * parent[i] is an ancestor-or-self of node i-1; label[i] is an int32 label
+ * id (equal ids are equal labels, unit costs). The reference passes the same
+ * trees as preorder children lists plus string labels
+ * (X-TED_GPU/TED_C++.h:27-28, README.md:31-44); the C++ drop-in converts.
+ *
+ * Tables: td is m x n row-major in preorder, td[i*n + j] =
+ * TED(subtree_x(i), subtree_y(j)); the distance is td[0].
+ */
+#ifndef XTED_H
+#define XTED_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define XTED_OK 0
+#define XTED_ERR_INVALID_ARG 1
+#define XTED_ERR_NOT_PREORDER 2
+#define XTED_ERR_OUT_OF_MEMORY 3
+#define XTED_ERR_DEVICE 4
+#define XTED_ERR_TOO_LARGE 5
+#define XTED_ERR_PARSE 6
+#define XTED_ERR_BUSY 7
+#define XTED_ERR_TIMEOUT 8
+#define XTED_ERR_NOT_FOUND 9
+#define XTED_ERR_CANCELLED 10
+
+/* Granularity classes (xted_force_class): 0 = lane per table,
+ * 1 + wi*5 + ci = workgroup row scan with 2^wi waves and 2^ci columns per
+ * lane (wi, ci in 0..4). */
+#define XTED_CLASS_AUTO (-1)
+#define XTED_CLASS_LANE 0
+
+typedef struct xted_context xted_context;
+
+typedef struct xted_stats {
+  double plan_ms;   /* device plan build incl. the bucket-count read-back */
+  double dp_ms;     /* forest-DP level loop (device time) */
+  double total_ms;  /* whole call, host inputs to host outputs */
+  int64_t cells;    /* interior forest cells, sum over pairs of S_x * S_y */
+  int64_t tables;   /* keyroot-pair tables */
+  int32_t levels;
+  int32_t launches; /* forest kernel launches */
+} xted_stats;
+
+/* ---- context --------------------------------------------------------------
+ * One context = one GPU, one HIP stream, device buffers reused across calls.
+ * Calls on one context are serialised; use one context per host thread for
+ * concurrency. The reference keeps equivalent state in globals
+ * (X-TED_GPU/parallel_X_TED.cu:59-65, X-TED_GPU/TED.cu:3). */
+int xted_create(int32_t device, xted_context** out);
+int xted_destroy(xted_context* ctx);
+const char* xted_last_error(void);
+const char* xted_status_string(int32_t status);
+
+/* ---- drop-in single pair ----------------------------------------------------
+ * Replaces standard_ted_1 / standard_ted (X-TED_GPU/TED.cu:78-128) and the
+ * scheduler behind them, parallel_standard_ted_test
+ * (X-TED_GPU/parallel_X_TED.cu:169-399). td (m*n ints) may be NULL.
+ * Empty trees follow the reference: m == 0 gives n, n == 0 gives m
+ * (X-TED_GPU/TED.cu:119-124). */
+int xted_ted(xted_context* ctx, const int32_t* x_parent, const int32_t* x_label, int32_t m,
+             const int32_t* y_parent, const int32_t* y_label, int32_t n, int32_t* td,
+             int32_t* dist, xted_stats* stats);
+
+/* ---- structural arrays --------------------------------------------------------
+ * Device-computed outermost-right leaves (orl, n ints), keyroots in
+ * descending preorder (kr, up to n ints, count in *K) and keyroot depth per
+ * keyroot (kr_depth, K ints). Replaces outermost_right_leaves
+ * (X-TED_GPU/TED.cu:21-42), key_roots (X-TED_GPU/TED.cu:44-75) and the
+ * keyroot depth loop (X-TED_GPU/parallel_X_TED.cu:184-212). Any output may be
+ * NULL. */
+int xted_tree_structure(xted_context* ctx, const int32_t* parent, int32_t n, int32_t* orl,
+                        int32_t* kr, int32_t* K, int32_t* kr_depth);
+
+/* ---- device-resident tree pool and batches ------------------------------------
+ * No counterpart in the reference, which evaluates one pair per call
+ * (X-TED_GPU/test_TED.cu:36-51). Trees are concatenated: tree t has sizes[t]
+ * nodes. ids receives the pool ids (may be NULL). */
+int xted_pool_add(xted_context* ctx, int32_t ntrees, const int32_t* sizes,
+                  const int32_t* parents, const int32_t* labels, int32_t* ids);
+int xted_pool_clear(xted_context* ctx);
+int xted_pool_size(xted_context* ctx, int32_t* ntrees);
+/* Distances of the pairs (x_ids[i], y_ids[i]) of pool trees, all pairs in one
+ * device plan. td_out (may be NULL; entries may be NULL) receives each pair's
+ * m x n table. */
+int xted_batch(xted_context* ctx, int32_t npairs, const int32_t* x_ids, const int32_t* y_ids,
+               int32_t* dist, int32_t* const* td_out, xted_stats* stats);
+
+/* ---- tuning and test hooks ------------------------------------------------------
+ * Granularity cut-overs, the role of the fixed thresholds 15 / 18,750 /
+ * 25,250,000 in X-TED_GPU/parallel_X_TED.cu:382. lds_per_wave: slot bytes
+ * one wave may own before a table gets more waves; tables with at least
+ * wide_cols columns get at least min_waves waves. */
+int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t min_waves,
+                    int32_t wide_cols);
+/* Forces every table that fits into class cls (XTED_CLASS_AUTO restores the
+ * rule); lane_enable = 0 sends single-row tables to the row-scan kernels. */
+int xted_force_class(xted_context* ctx, int32_t cls, int32_t lane_enable);
+
+/* ---- per-class device timing ---------------------------------------------------
+ * With profiling on, xted_ted / xted_batch bracket every forest-DP launch
+ * with events on the context's stream; xted_class_profile reports, for class
+ * cls of the last call, the launches, tables, interior cells and summed
+ * device milliseconds. The reference times its kernels the same way
+ * (X-TED_GPU/parallel_X_TED.cu:429-443). */
+int xted_set_profile(xted_context* ctx, int32_t on);
+int xted_class_profile(xted_context* ctx, int32_t cls, int64_t* launches, int64_t* tables,
+                       int64_t* cells, double* ms);
+
+/* ---- continuous-batching server ---------------------------------------------------
+ * No counterpart in the reference. A server thread drains submitted queries
+ * over pool trees into device batches, admitting each by the HBM its TD and
+ * TD^T tables need (8*m*n bytes) against hbm_budget bytes. */
+typedef struct xted_server xted_server;
+typedef struct xted_result {
+  uint64_t tag;         /* caller's tag from submit */
+  int32_t status;       /* XTED_OK or an error code */
+  int32_t dist;
+  double latency_ms;    /* submit -> result available */
+  double queue_ms;      /* submit -> admitted into a device batch */
+} xted_result;
+int xted_server_create(xted_context* ctx, int64_t hbm_budget, int32_t max_batch,
+                       xted_server** out);
+int xted_server_submit(xted_server* srv, int32_t x_id, int32_t y_id, uint64_t tag);
+/* Copies up to cap finished results, waiting up to timeout_ms for at least
+ * one (0 = do not wait). *count receives the number copied. */
+int xted_server_poll(xted_server* srv, xted_result* out, int32_t cap, int32_t timeout_ms,
+                     int32_t* count);
+int xted_server_destroy(xted_server* srv);
+
+/* ---- host utilities (no GPU needed) ---------------------------------------------------
+ * Seeded synthetic trees: kind 0 = random recursive tree, 1 = deep/narrow,
+ * 2 = shallow/wide; depth_min/depth_max > 0 reject trees outside the band. */
+int xted_generate(int32_t kind, int32_t n, int32_t alphabet, uint64_t seed, int32_t depth_min,
+                  int32_t depth_max, int32_t* parent, int32_t* label);
+uint64_t xted_derive_seed(uint64_t config, uint64_t index);
+int xted_validate(const int32_t* parent, int32_t n);
+/* Host restatements used for planning and reporting: S = sum of keyroot
+ * subtree sizes (cells of a pair = S_x * S_y) and the depth. */
+int xted_tree_stats(const int32_t* parent, int32_t n, int64_t* ksum, int32_t* depth,
+                    int32_t* nkeyroots);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,169 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle, bit-exact
+on distances and full tree-distance tables."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import xted
+from conftest import GOLDEN, golden_pairs, load_dataset_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def gen(kind, n, alphabet, cfg, idx, band=(0, 0)):
+    return xted.generate(kind, n, alphabet, xted.derive_seed(cfg, idx), *band)
+
+
+def assert_same(ctx, x, y, threads=8):
+    d, td, _ = ctx.ted(x, y)
+    od, otd, _ = oracle.ted(x, y, threads=threads)
+    assert d == od
+    assert td.shape == otd.shape
+    mism = np.argwhere(td != otd)
+    assert len(mism) == 0, f"{len(mism)} TD cells differ, first {mism[:5].tolist()}"
+    return d
+
+
+@pytest.mark.parametrize("case", golden_pairs(), ids=lambda c: c[0])
+def test_golden_reference_tables(gpu_ctx, case):
+    name, x, y, td, dist, _ = case
+    d, t, _ = gpu_ctx.ted(x, y)
+    assert d == dist
+    assert (t == td).all()
+
+
+def test_config1_rrt100(gpu_ctx):
+    """100-node random recursive trees, alphabet 50: distance + 100x100 table."""
+    for i in range(12):
+        x = gen(xted.GEN_RRT, 100, 50, 1, 2 * i)
+        y = gen(xted.GEN_RRT, 100, 50, 1, 2 * i + 1)
+        assert_same(gpu_ctx, x, y)
+
+
+@pytest.mark.parametrize("kind", [xted.GEN_DEEP, xted.GEN_SHALLOW])
+@pytest.mark.parametrize("alphabet", [200, 1000])
+def test_config2_1000_nodes(gpu_ctx, kind, alphabet):
+    band = (60, 120) if kind == xted.GEN_DEEP else (0, 0)
+    x = gen(kind, 1000, alphabet, 2, 0, band)
+    y = gen(kind, 1000, alphabet, 2, 1, band)
+    assert_same(gpu_ctx, x, y, threads=16)
+
+
+def test_config2_mixed_shapes(gpu_ctx):
+    x = gen(xted.GEN_DEEP, 1000, 200, 2, 10, (60, 120))
+    y = gen(xted.GEN_SHALLOW, 1000, 200, 2, 11)
+    assert_same(gpu_ctx, x, y, threads=16)
+    assert_same(gpu_ctx, y, x, threads=16)
+
+
+@pytest.mark.parametrize("cls", list(range(1, 26)))
+def test_every_row_class_forced(gpu_ctx, cls):
+    """Every table that fits goes through one (waves, columns-per-lane) class;
+    single-row tables too (lane kernel off). Results must not change a bit."""
+    gpu_ctx.force_class(cls, lane_enable=False)
+    try:
+        for i, (kx, ky, n1, n2) in enumerate([(0, 0, 90, 70), (1, 2, 120, 150), (2, 1, 80, 130),
+                                              (1, 1, 140, 110)]):
+            x = gen(kx, n1, 5, 30 + cls, 2 * i)
+            y = gen(ky, n2, 5, 30 + cls, 2 * i + 1)
+            assert_same(gpu_ctx, x, y)
+    finally:
+        gpu_ctx.force_class(xted.CLASS_AUTO, lane_enable=True)
+
+
+def test_lane_kernel_and_rowscan_agree(gpu_ctx):
+    x = gen(xted.GEN_SHALLOW, 300, 7, 40, 0)
+    y = gen(xted.GEN_RRT, 250, 7, 40, 1)
+    d1, t1, _ = gpu_ctx.ted(x, y)
+    gpu_ctx.force_class(xted.CLASS_AUTO, lane_enable=False)
+    try:
+        d2, t2, _ = gpu_ctx.ted(x, y)
+    finally:
+        gpu_ctx.force_class(xted.CLASS_AUTO, lane_enable=True)
+    assert d1 == d2 and (t1 == t2).all()
+
+
+def test_dataset_known_answers(gpu_ctx):
+    with open(os.path.join(GOLDEN, "datasets", "expected.json")) as f:
+        expected = json.load(f)
+    for name, dist in sorted(expected.items()):
+        x, y = load_dataset_pair(name)
+        d, _, _ = gpu_ctx.ted(x, y, want_td=False)
+        assert d == dist, name
+
+
+def test_dataset_full_tables(gpu_ctx):
+    for name in ["swissport_100", "python_100", "dblp_100", "swissport_1000"]:
+        x, y = load_dataset_pair(name)
+        assert_same(gpu_ctx, x, y)
+
+
+def test_edge_cases(gpu_ctx):
+    one_a = (np.array([-1], np.int32), np.array([1], np.int32))
+    one_b = (np.array([-1], np.int32), np.array([2], np.int32))
+    assert gpu_ctx.ted(one_a, one_a)[0] == 0
+    assert gpu_ctx.ted(one_a, one_b)[0] == 1
+    empty = (np.zeros(0, np.int32), np.zeros(0, np.int32))
+    star = (np.array([-1, 0, 0, 0], np.int32), np.array([1, 2, 3, 2], np.int32))
+    assert gpu_ctx.ted(empty, star)[0] == 4
+    assert gpu_ctx.ted(star, empty)[0] == 4
+    # single node against a star holding its label (root onto leaf)
+    one_3 = (np.array([-1], np.int32), np.array([3], np.int32))
+    assert gpu_ctx.ted(one_3, star)[0] == 3
+    path = (np.arange(-1, 199, dtype=np.int32), np.arange(200, dtype=np.int32) % 5)
+    wide = (np.array([-1] + [0] * 299, np.int32), np.arange(300, dtype=np.int32) % 4)
+    for a in (one_a, star, path, wide):
+        for b in (one_b, star, path, wide):
+            assert_same(gpu_ctx, a, b)
+    # identical trees: distance 0 and a zero diagonal
+    x = gen(xted.GEN_RRT, 300, 9, 50, 0)
+    d, td, _ = gpu_ctx.ted(x, x)
+    assert d == 0 and (np.diag(td) == 0).all()
+
+
+def test_structure_arrays(gpu_ctx):
+    for kind in (0, 1, 2):
+        for n in (1, 2, 5, 300, 2000):
+            p, _ = gen(kind, n, 3, 60 + kind, n)
+            orl, kr, kd = gpu_ctx.tree_structure(p)
+            o = oracle.orl(p)
+            assert (orl == o).all()
+            k = oracle.keyroots(o)
+            assert (kr == k).all()
+            assert (kd == oracle.keyroot_depth(o, k)).all()
+
+
+def test_invalid_inputs(gpu_ctx):
+    bad = (np.array([-1, 0, 0, 1], np.int32), np.zeros(4, np.int32))
+    ok = (np.array([-1, 0], np.int32), np.zeros(2, np.int32))
+    with pytest.raises(xted.XtedError) as e:
+        gpu_ctx.ted(bad, ok)
+    assert e.value.status == xted.ERR_NOT_PREORDER
+    big = (np.arange(-1, 40000, dtype=np.int32), np.zeros(40001, np.int32))
+    with pytest.raises(xted.XtedError) as e:
+        gpu_ctx.ted(big, ok)
+    assert e.value.status == xted.ERR_TOO_LARGE
+    # the context still works after errors
+    assert gpu_ctx.ted(ok, ok)[0] == 0
+
+
+def test_batch_matches_single_and_oracle(gpu_ctx):
+    rng = np.random.default_rng(3)
+    trees = []
+    for i in range(24):
+        kind = int(rng.integers(0, 3))
+        trees.append(gen(kind, int(rng.integers(1, 400)), 20, 70, i))
+    gpu_ctx.pool_clear()
+    ids = gpu_ctx.pool_add(trees)
+    xs = rng.integers(0, 24, 60).astype(np.int32)
+    ys = rng.integers(0, 24, 60).astype(np.int32)
+    shapes = [(len(trees[a][0]), len(trees[b][0])) for a, b in zip(xs, ys)]
+    dist, tds, st = gpu_ctx.batch(ids[xs], ids[ys], td_shapes=shapes)
+    assert st.tables > 0 and st.cells > 0
+    for k, (a, b) in enumerate(zip(xs, ys)):
+        od, otd, _ = oracle.ted(trees[a], trees[b], threads=4)
+        assert dist[k] == od
+        assert (tds[k] == otd).all()

@@ -1,0 +1,311 @@
+// C ABI implementation (include/xted.h) over the host engine.
+#include "../../include/xted.h"
+
+#include <algorithm>
+#include <exception>
+#include <new>
+#include <vector>
+
+#include "engine.hpp"
+#include "server.hpp"
+#include "trees.hpp"
+
+using namespace xted;
+
+struct xted_context {
+  Context ctx;
+  explicit xted_context(int d) : ctx(d) {}
+};
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return ERR_INVALID_ARG;
+  } catch (...) {
+    set_error("unknown failure");
+    return ERR_INVALID_ARG;
+  }
+}
+
+void to_c(const BatchStats& s, xted_stats* out) {
+  if (!out) return;
+  out->plan_ms = s.plan_ms;
+  out->dp_ms = s.dp_ms;
+  out->total_ms = s.total_ms;
+  out->cells = s.cells;
+  out->tables = s.tables;
+  out->levels = s.levels;
+  out->launches = s.launches;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xted_create(int32_t device, xted_context** out) {
+  return guarded([&]() -> int {
+    if (!out) {
+      set_error("xted_create: null output");
+      return ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    auto* c = new xted_context(device);
+    int st = c->ctx.init();
+    if (st != OK) {
+      delete c;
+      return st;
+    }
+    *out = c;
+    return OK;
+  });
+}
+
+int xted_destroy(xted_context* ctx) {
+  delete ctx;
+  return OK;
+}
+
+const char* xted_last_error(void) { return last_error(); }
+
+const char* xted_status_string(int32_t status) {
+  switch (status) {
+    case OK: return "ok";
+    case ERR_INVALID_ARG: return "invalid argument";
+    case ERR_NOT_PREORDER: return "tree is not a preorder parent array";
+    case ERR_OUT_OF_MEMORY: return "out of memory";
+    case ERR_DEVICE: return "device error";
+    case ERR_TOO_LARGE: return "input too large";
+    case ERR_PARSE: return "parse error";
+    case ERR_BUSY: return "busy";
+    case ERR_TIMEOUT: return "timeout";
+    case ERR_NOT_FOUND: return "not found";
+    case ERR_CANCELLED: return "cancelled";
+    default: return "unknown status";
+  }
+}
+
+int xted_ted(xted_context* ctx, const int32_t* x_parent, const int32_t* x_label, int32_t m,
+             const int32_t* y_parent, const int32_t* y_label, int32_t n, int32_t* td,
+             int32_t* dist, xted_stats* stats) {
+  return guarded([&]() -> int {
+    if (!ctx || !dist || m < 0 || n < 0) {
+      set_error("xted_ted: invalid argument");
+      return ERR_INVALID_ARG;
+    }
+    if (stats) *stats = xted_stats{};
+    // X-TED_GPU/TED.cu:119-124
+    if (m == 0) {
+      *dist = n;
+      return OK;
+    }
+    if (n == 0) {
+      *dist = m;
+      return OK;
+    }
+    if (!x_parent || !x_label || !y_parent || !y_label) {
+      set_error("xted_ted: null tree array");
+      return ERR_INVALID_ARG;
+    }
+    Context& c = ctx->ctx;
+    int st = c.pool_clear();
+    if (st != OK) return st;
+    std::vector<int32_t> sizes{m, n};
+    std::vector<int32_t> par(m + n), lab(m + n);
+    std::copy(x_parent, x_parent + m, par.begin());
+    std::copy(y_parent, y_parent + n, par.begin() + m);
+    std::copy(x_label, x_label + m, lab.begin());
+    std::copy(y_label, y_label + n, lab.begin() + m);
+    int32_t ids[2];
+    st = c.pool_add(2, sizes.data(), par.data(), lab.data(), ids);
+    if (st != OK) return st;
+    int32_t* tds[1] = {td};
+    BatchStats bs;
+    st = c.run_batch(1, &ids[0], &ids[1], dist, td ? tds : nullptr, &bs);
+    to_c(bs, stats);
+    return st;
+  });
+}
+
+int xted_tree_structure(xted_context* ctx, const int32_t* parent, int32_t n, int32_t* orl,
+                        int32_t* kr, int32_t* K, int32_t* kr_depth) {
+  return guarded([&]() -> int {
+    if (!ctx || !parent || n <= 0) {
+      set_error("xted_tree_structure: invalid argument");
+      return ERR_INVALID_ARG;
+    }
+    Context& c = ctx->ctx;
+    int st = c.pool_clear();
+    if (st != OK) return st;
+    std::vector<int32_t> lab(n, 0);
+    int32_t id = -1;
+    st = c.pool_add(1, &n, parent, lab.data(), &id);
+    if (st != OK) return st;
+    return c.tree_arrays(id, orl, kr, K, kr_depth);
+  });
+}
+
+int xted_pool_add(xted_context* ctx, int32_t ntrees, const int32_t* sizes,
+                  const int32_t* parents, const int32_t* labels, int32_t* ids) {
+  return guarded([&]() -> int {
+    if (!ctx) {
+      set_error("xted_pool_add: null context");
+      return ERR_INVALID_ARG;
+    }
+    return ctx->ctx.pool_add(ntrees, sizes, parents, labels, ids);
+  });
+}
+
+int xted_pool_clear(xted_context* ctx) {
+  if (!ctx) return ERR_INVALID_ARG;
+  return ctx->ctx.pool_clear();
+}
+
+int xted_pool_size(xted_context* ctx, int32_t* ntrees) {
+  if (!ctx || !ntrees) return ERR_INVALID_ARG;
+  *ntrees = ctx->ctx.pool_size();
+  return OK;
+}
+
+int xted_batch(xted_context* ctx, int32_t npairs, const int32_t* x_ids, const int32_t* y_ids,
+               int32_t* dist, int32_t* const* td_out, xted_stats* stats) {
+  return guarded([&]() -> int {
+    if (!ctx) {
+      set_error("xted_batch: null context");
+      return ERR_INVALID_ARG;
+    }
+    BatchStats bs;
+    int st = ctx->ctx.run_batch(npairs, x_ids, y_ids, dist, td_out, &bs);
+    to_c(bs, stats);
+    return st;
+  });
+}
+
+int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t min_waves,
+                    int32_t wide_cols) {
+  if (!ctx || lds_per_wave < 0 || min_waves < 1 || min_waves > kMaxRowWaves || wide_cols < 0) {
+    set_error("xted_set_tuning: invalid argument");
+    return ERR_INVALID_ARG;
+  }
+  ctx->ctx.rule.lds_per_wave = lds_per_wave;
+  ctx->ctx.rule.min_waves = min_waves;
+  ctx->ctx.rule.wide_cols = wide_cols;
+  return OK;
+}
+
+int xted_force_class(xted_context* ctx, int32_t cls, int32_t lane_enable) {
+  if (!ctx || cls < -1 || cls > kNumRowClasses) {
+    set_error("xted_force_class: invalid class");
+    return ERR_INVALID_ARG;
+  }
+  ctx->ctx.force_class = cls;
+  ctx->ctx.lane_enable = lane_enable ? 1 : 0;
+  return OK;
+}
+
+int xted_set_profile(xted_context* ctx, int32_t on) {
+  if (!ctx) return ERR_INVALID_ARG;
+  ctx->ctx.profile = on ? 1 : 0;
+  return OK;
+}
+
+int xted_class_profile(xted_context* ctx, int32_t cls, int64_t* launches, int64_t* tables,
+                       int64_t* cells, double* ms) {
+  if (!ctx || cls < 0 || cls >= kNumClasses) {
+    set_error("xted_class_profile: invalid class");
+    return ERR_INVALID_ARG;
+  }
+  const ClassProfile& cp = ctx->ctx.class_profile[cls];
+  if (launches) *launches = cp.launches;
+  if (tables) *tables = cp.tables;
+  if (cells) *cells = cp.cells;
+  if (ms) *ms = cp.ms;
+  return OK;
+}
+
+int xted_server_create(xted_context* ctx, int64_t hbm_budget, int32_t max_batch,
+                       xted_server** out) {
+  return guarded([&]() -> int {
+    if (!ctx || !out || hbm_budget <= 0 || max_batch <= 0) {
+      set_error("xted_server_create: invalid argument");
+      return ERR_INVALID_ARG;
+    }
+    *out = reinterpret_cast<xted_server*>(new Server(&ctx->ctx, hbm_budget, max_batch));
+    return OK;
+  });
+}
+
+int xted_server_submit(xted_server* srv, int32_t x_id, int32_t y_id, uint64_t tag) {
+  return guarded([&]() -> int {
+    if (!srv) return ERR_INVALID_ARG;
+    return reinterpret_cast<Server*>(srv)->submit(x_id, y_id, tag);
+  });
+}
+
+int xted_server_poll(xted_server* srv, xted_result* out, int32_t cap, int32_t timeout_ms,
+                     int32_t* count) {
+  return guarded([&]() -> int {
+    if (!srv || !count || (cap > 0 && !out)) return ERR_INVALID_ARG;
+    std::vector<ServerResult> rs;
+    int st = reinterpret_cast<Server*>(srv)->poll(cap, timeout_ms, rs);
+    for (size_t i = 0; i < rs.size(); ++i) {
+      out[i].tag = rs[i].tag;
+      out[i].status = rs[i].status;
+      out[i].dist = rs[i].dist;
+      out[i].latency_ms = rs[i].latency_ms;
+      out[i].queue_ms = rs[i].queue_ms;
+    }
+    *count = (int32_t)rs.size();
+    return st;
+  });
+}
+
+int xted_server_destroy(xted_server* srv) {
+  delete reinterpret_cast<Server*>(srv);
+  return OK;
+}
+
+int xted_generate(int32_t kind, int32_t n, int32_t alphabet, uint64_t seed, int32_t depth_min,
+                  int32_t depth_max, int32_t* parent, int32_t* label) {
+  return guarded([&]() -> int {
+    if (n <= 0 || !parent || !label || kind < 0 || kind > 2) {
+      set_error("xted_generate: invalid argument");
+      return ERR_INVALID_ARG;
+    }
+    Tree t = (depth_min > 0 || depth_max > 0)
+                 ? generate_banded(kind, n, alphabet, seed, depth_min,
+                                   depth_max > 0 ? depth_max : n)
+                 : generate(kind, n, alphabet, seed);
+    std::copy(t.parent.begin(), t.parent.end(), parent);
+    std::copy(t.label.begin(), t.label.end(), label);
+    return OK;
+  });
+}
+
+uint64_t xted_derive_seed(uint64_t config, uint64_t index) { return derive_seed(config, index); }
+
+int xted_validate(const int32_t* parent, int32_t n) { return validate_preorder(parent, n); }
+
+int xted_tree_stats(const int32_t* parent, int32_t n, int64_t* ksum, int32_t* depth,
+                    int32_t* nkeyroots) {
+  return guarded([&]() -> int {
+    int st = validate_preorder(parent, n);
+    if (st != OK) return st;
+    std::vector<int32_t> orl(n);
+    compute_orl(parent, n, orl.data());
+    auto kr = keyroots_desc(parent, orl.data(), n);
+    if (ksum) *ksum = keyroot_size_sum(orl.data(), kr);
+    if (depth) *depth = tree_depth(parent, n);
+    if (nkeyroots) *nkeyroots = (int32_t)kr.size();
+    return OK;
+  });
+}
+
+}  // extern "C"

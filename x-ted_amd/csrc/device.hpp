@@ -1,0 +1,113 @@
+// Device-side data layout shared by the HIP kernels and the host engine.
+//
+// Tree pool (structure of arrays, all int32, one region of n+1 entries per
+// tree starting at node_off):
+//   node arrays    parent/label/orl/kdep/kdn  [node_off + i], i in [0, n)
+//   leaf rank      lrank                      [node_off + i], i in [0, n]
+//   keyroot arrays kr/krkd/krsize/krleaf      [node_off + k], k in [0, K)
+//                  (keyroots in descending preorder; K <= n)
+// A pair owns a tree-distance table TD (m x n, row-major, preorder) and its
+// transpose TDT (n x m); both live in one int32 arena at td_off / tdt_off.
+#pragma once
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#define XTED_HD __host__ __device__
+#else
+#define XTED_HD
+#endif
+
+namespace xted {
+
+struct TreeMeta {
+  int64_t node_off;  // offset of the tree's region (n+1 entries)
+  int32_t n;         // node count
+  int32_t K;         // keyroot count
+  int32_t kdmax;     // keyroot nesting height of the root
+  int32_t nleaves;   // leaf count
+};
+
+struct PairMeta {
+  int32_t tx, ty;   // tree ids (rows = x, columns = y)
+  int64_t td_off;   // TD  offset (m*n ints)
+  int64_t tdt_off;  // TDT offset (n*m ints)
+};
+
+// Pointers to the pool arrays (device addresses).
+struct PoolView {
+  const int32_t* parent;
+  const int32_t* label;
+  const int32_t* orl;
+  const int32_t* kdep;   // nesting depth of the keyroot that owns node's right path
+  const int32_t* kdn;    // keyroot nesting height at keyroots, -1 elsewhere
+  const int32_t* lrank;  // leaves strictly before node (n+1 entries per tree)
+  const int32_t* kr;     // keyroot node ids, descending
+  const int32_t* krkd;   // nesting height per keyroot
+  const int32_t* krsize; // subtree size per keyroot
+  const int32_t* krleaf; // leaves in subtree per keyroot
+  const TreeMeta* trees;
+};
+
+// A table task: pair id (24 bits) | x keyroot node (20 bits) | y keyroot node (20 bits).
+// Trees in a pool are limited to 2^20 nodes and a batch to 2^24 pairs.
+constexpr int kTaskNodeBits = 20;
+constexpr int kTaskPairBits = 24;
+XTED_HD inline uint64_t task_pack(uint32_t pair, uint32_t i0, uint32_t j0) {
+  return ((uint64_t)pair << 40) | ((uint64_t)i0 << 20) | (uint64_t)j0;
+}
+XTED_HD inline uint32_t task_pair(uint64_t t) { return (uint32_t)(t >> 40); }
+XTED_HD inline uint32_t task_i0(uint64_t t) { return (uint32_t)(t >> 20) & 0xFFFFFu; }
+XTED_HD inline uint32_t task_j0(uint64_t t) { return (uint32_t)t & 0xFFFFFu; }
+
+// ---- granularity classes ----------------------------------------------------
+// Class 0: lane-per-table (a table whose short side is one node: one row).
+// Classes 1..25: workgroup row scan with W = 2^wi waves and C = 2^ci columns
+//   per lane, class = 1 + wi*5 + ci (wi, ci in 0..4).
+// Class 26: tables too wide or too LDS-hungry for the on-chip form.
+constexpr int kClassLane = 0;
+constexpr int kNumRowClasses = 25;
+constexpr int kClassXL = 26;
+constexpr int kNumClasses = 27;
+XTED_HD inline int row_class(int wi, int ci) { return 1 + wi * 5 + ci; }
+
+// Saved-row slots are uint16 (F <= |subtree x| + |subtree y| < 65536).
+// LDS budget for the slots of one table, per wave of its workgroup.
+constexpr int kLdsPerWave = 10 * 1024;
+constexpr int kLdsMaxTable = 150 * 1024;
+constexpr int kMaxRowWaves = 16;
+constexpr int kMaxCols = 64 * 16 * kMaxRowWaves;  // 16384
+
+// Tuning knobs carried by the plan builder (autotuner writes these).
+struct ClassRule {
+  int32_t lds_per_wave;  // bytes of slot storage one wave may own
+  int32_t min_waves;     // lower bound on W for tables with >= wide_cols columns
+  int32_t wide_cols;
+  int32_t lds_max;       // slot bytes one workgroup may hold (device limit)
+};
+
+XTED_HD inline int ilog2_ceil(int v) {
+  int r = 0;
+  while ((1 << r) < v) ++r;
+  return r;
+}
+
+// Class of table (a x b) given the LDS need of its oriented form.
+XTED_HD inline int table_class(int a, int b, int64_t lds_need, const ClassRule& rule) {
+  const int rows = a <= b ? a : b;
+  const int cols = a <= b ? b : a;
+  if (rows == 1) return kClassLane;
+  if (cols > kMaxCols || lds_need > rule.lds_max) return kClassXL;
+  int wi = 0;
+  while (wi < 4 && lds_need > (int64_t)rule.lds_per_wave * (1 << wi)) ++wi;
+  while (wi < 4 && cols > 64 * 16 * (1 << wi)) ++wi;
+  if (cols >= rule.wide_cols) {
+    while (wi < 4 && (1 << wi) < rule.min_waves) ++wi;
+  }
+  int w = 1 << wi;
+  int c = (cols + 64 * w - 1) / (64 * w);
+  int ci = ilog2_ceil(c);
+  if (ci > 4) return kClassXL;
+  return row_class(wi, ci);
+}
+
+}  // namespace xted

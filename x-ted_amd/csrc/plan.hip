@@ -1,0 +1,180 @@
+// Device-side level scheduler: every keyroot-pair table of every pair in a
+// batch is assigned level = depth_x(k) + depth_y(l) and a granularity class,
+// then scattered into (level, class) buckets, all on the device.
+//
+// Replaces the reference's per-level host loop (X-TED_GPU/parallel_X_TED.cu:
+// 214-220 level array, :304-385 fetch_task / fetch_size_queue /
+// thrust::sort_by_key / filter_new with several host syncs per level, and
+// X-TED_GPU/function.cu:18-57). Here the whole plan is built once per batch
+// by three kernels (count, scan, scatter) and read back with one copy of the
+// bucket counts. Appends use wave-aggregated atomics: each distinct bucket in
+// a wave costs one atomic, and a wave's tables land contiguously.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace xted {
+
+constexpr int kPlanThreads = 256;
+
+__device__ __forceinline__ int find_pair(const int64_t* blk_start, int npairs, int64_t b) {
+  int lo = 0, hi = npairs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (blk_start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Per-table classification shared by both passes.
+__device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
+                                         const TreeMeta& ty, int k, int l, int* bucket,
+                                         int* lds, uint32_t* i0, uint32_t* j0,
+                                         unsigned long long* cells) {
+  const int64_t kx = tx.node_off + k, ly = ty.node_off + l;
+  const int32_t a = A.pool.krsize[kx], b = A.pool.krsize[ly];
+  *cells = (unsigned long long)a * (unsigned long long)b;
+  const int32_t kdx = A.pool.krkd[kx], kdy = A.pool.krkd[ly];
+  const int32_t level = kdx + kdy;
+  int64_t need = (a <= b) ? (int64_t)kdx * (A.pool.krleaf[ly] - 1) * 2
+                          : (int64_t)kdy * (A.pool.krleaf[kx] - 1) * 2;
+  int cls = table_class(a, b, need, A.rule);
+  if (cls == kClassLane && !A.lane_enable) {
+    // single-row tables routed to the narrowest row-scan class that fits
+    const int cols = a <= b ? b : a;
+    int c = (cols + 63) / 64, ci = ilog2_ceil(c);
+    cls = ci > 4 ? kClassXL : row_class(0, ci);
+  }
+  if (A.force_class >= 1 && cls >= 1 && cls <= kNumRowClasses) {
+    const int wi = (A.force_class - 1) / 5, ci = (A.force_class - 1) % 5;
+    const int cols = a <= b ? b : a;
+    if (cols <= 64 * (1 << wi) * (1 << ci) && need <= A.rule.lds_max) cls = A.force_class;
+  }
+  *bucket = level * kNumClasses + cls;
+  *lds = (int)need;
+  *i0 = (uint32_t)A.pool.kr[kx];
+  *j0 = (uint32_t)A.pool.kr[ly];
+}
+
+// Returns this lane's slot in cursors[bucket] after one wave-aggregated
+// atomicAdd per distinct bucket among the active lanes. With sums != null it
+// also adds the lanes' values of each bucket into sums[bucket].
+__device__ __forceinline__ unsigned long long wave_reserve(unsigned long long* cursors,
+                                                           unsigned long long* sums,
+                                                           int bucket, bool active,
+                                                           unsigned long long value) {
+  unsigned long long pos = 0;
+  const unsigned long long lt = __lanemask_lt();
+  unsigned long long todo = __ballot(active);
+  const int lane = (int)(threadIdx.x & 63);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lb = __shfl(bucket, leader);
+    const bool mine = active && bucket == lb;
+    const unsigned long long same = __ballot(mine);
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(&cursors[lb], (unsigned long long)__popcll(same));
+    base = __shfl(base, leader);
+    if (mine) pos = base + __popcll(same & lt);
+    if (sums) {
+      unsigned long long v = mine ? value : 0ull;
+      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+      if (lane == leader) atomicAdd(&sums[lb], v);
+    }
+    todo &= ~same;
+  }
+  return pos;
+}
+
+template <bool kScatter>
+__global__ __launch_bounds__(kPlanThreads) void plan_kernel(PlanArgs A) {
+  const int64_t blk = blockIdx.x;
+  const int p = find_pair(A.blk_start, A.npairs, blk);
+  const PairMeta pm = A.pairs[p];
+  const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
+  const int Kx = tx.K, Ky = ty.K;
+  const int lper = max(1, A.lchunk_tables / max(Kx, 1));  // same as plan_blocks()
+  const int l_lo = (int)(blk - A.blk_start[p]) * lper;
+  const int l_hi = min(Ky, l_lo + lper);
+  const int64_t total = (int64_t)max(0, l_hi - l_lo) * Kx;
+  for (int64_t base = 0; base < total; base += kPlanThreads) {
+    const int64_t idx = base + threadIdx.x;
+    const bool active = idx < total;
+    int bucket = 0, lds = 0;
+    uint32_t i0 = 0, j0 = 0;
+    unsigned long long cells = 0;
+    if (active) {
+      const int l = l_lo + (int)(idx / Kx), k = (int)(idx % Kx);
+      classify(A, tx, ty, k, l, &bucket, &lds, &i0, &j0, &cells);
+    }
+    if (kScatter) {
+      const unsigned long long pos = wave_reserve(A.cursor, nullptr, bucket, active, 0);
+      if (active) A.tasks[pos] = task_pack((uint32_t)p, i0, j0);
+    } else {
+      (void)wave_reserve(A.counts, A.cells, bucket, active, cells);
+      if (active && (bucket % kNumClasses) != kClassLane) atomicMax(&A.maxlds[bucket % kNumClasses], lds);
+    }
+  }
+}
+
+// Exclusive scan of the bucket counts into the scatter cursors (one block).
+__global__ __launch_bounds__(1024) void plan_scan_kernel(const unsigned long long* counts,
+                                                          unsigned long long* cursor, int nb) {
+  __shared__ unsigned long long part[1024];
+  const int tid = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int lo = min(nb, tid * per), hi = min(nb, lo + per);
+  unsigned long long s = 0;
+  for (int i = lo; i < hi; ++i) s += counts[i];
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    unsigned long long v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  unsigned long long run = tid ? part[tid - 1] : 0;
+  for (int i = lo; i < hi; ++i) {
+    cursor[i] = run;
+    run += counts[i];
+  }
+}
+
+// Clears the count / max arrays (a kernel rather than a memset so the plan
+// can be captured into a graph).
+__global__ void plan_clear_kernel(unsigned long long* counts, unsigned long long* cells, int nb,
+                                  int32_t* maxlds) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nb) {
+    counts[i] = 0;
+    cells[i] = 0;
+  }
+  if (i < kNumClasses) maxlds[i] = 0;
+}
+
+void launch_plan(const PlanArgs& a, int64_t nblocks, hipStream_t s) {
+  const int nb = a.nlevels * kNumClasses;
+  hipLaunchKernelGGL(plan_clear_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, a.counts,
+                     a.cells, nb, a.maxlds);
+  hipLaunchKernelGGL(plan_kernel<false>, dim3((unsigned)nblocks), dim3(kPlanThreads), 0, s, a);
+  hipLaunchKernelGGL(plan_scan_kernel, dim3(1), dim3(1024), 0, s, a.counts, a.cursor, nb);
+  hipLaunchKernelGGL(plan_kernel<true>, dim3((unsigned)nblocks), dim3(kPlanThreads), 0, s, a);
+}
+
+// Distances TD[0][0] of every pair into dist[].
+__global__ void gather_dist_kernel(const PairMeta* pairs, const int32_t* td, int32_t npairs,
+                                   int32_t* dist) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < npairs) dist[i] = td[pairs[i].td_off];
+}
+
+void launch_gather_dist(const PairMeta* pairs, const int32_t* td, int32_t npairs, int32_t* dist,
+                        hipStream_t s) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(gather_dist_kernel, dim3((npairs + 255) / 256), dim3(256), 0, s, pairs, td,
+                     npairs, dist);
+}
+
+}  // namespace xted

@@ -1,0 +1,306 @@
+"""Python binding (ctypes) of the xted C ABI (include/xted.h).
+
+Used by the tests and bench.py. Loads x-ted_amd/lib/libxted.so, the library
+built in-tree by ``make -C x-ted_amd``; there is no fallback path: a missing
+library raises at import of the first call.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libxted.so")
+
+OK = 0
+ERR_INVALID_ARG = 1
+ERR_NOT_PREORDER = 2
+ERR_OUT_OF_MEMORY = 3
+ERR_DEVICE = 4
+ERR_TOO_LARGE = 5
+ERR_PARSE = 6
+ERR_BUSY = 7
+ERR_TIMEOUT = 8
+ERR_NOT_FOUND = 9
+ERR_CANCELLED = 10
+
+CLASS_AUTO = -1
+CLASS_LANE = 0
+NUM_CLASSES = 27
+
+
+def class_name(cls):
+    """'lane' or 'rowscan_w{W}_c{C}' (W waves, C columns per lane) or 'xl'."""
+    if cls == 0:
+        return "lane"
+    if 1 <= cls <= 25:
+        wi, ci = divmod(cls - 1, 5)
+        return f"rowscan_w{1 << wi}_c{1 << ci}"
+    return "xl"
+
+GEN_RRT, GEN_DEEP, GEN_SHALLOW = 0, 1, 2
+
+_I32P = ctypes.POINTER(ctypes.c_int32)
+
+# Every function the header declares, with (restype, argtypes).
+_SIGNATURES = {
+    "xted_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "xted_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "xted_last_error": (ctypes.c_char_p, []),
+    "xted_status_string": (ctypes.c_char_p, [ctypes.c_int32]),
+    "xted_ted": (ctypes.c_int, [ctypes.c_void_p, _I32P, _I32P, ctypes.c_int32, _I32P, _I32P,
+                                ctypes.c_int32, _I32P, _I32P, ctypes.c_void_p]),
+    "xted_tree_structure": (ctypes.c_int, [ctypes.c_void_p, _I32P, ctypes.c_int32, _I32P, _I32P,
+                                           _I32P, _I32P]),
+    "xted_pool_add": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32P, _I32P, _I32P, _I32P]),
+    "xted_pool_clear": (ctypes.c_int, [ctypes.c_void_p]),
+    "xted_pool_size": (ctypes.c_int, [ctypes.c_void_p, _I32P]),
+    "xted_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32P, _I32P, _I32P,
+                                  ctypes.c_void_p, ctypes.c_void_p]),
+    "xted_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32]),
+    "xted_force_class": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "xted_set_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "xted_class_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_int64),
+                                          ctypes.POINTER(ctypes.c_int64),
+                                          ctypes.POINTER(ctypes.c_int64),
+                                          ctypes.POINTER(ctypes.c_double)]),
+    "xted_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+    "xted_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_uint64]),
+    "xted_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                        ctypes.c_int32, _I32P]),
+    "xted_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "xted_generate": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, _I32P,
+                                     _I32P]),
+    "xted_derive_seed": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64]),
+    "xted_validate": (ctypes.c_int, [_I32P, ctypes.c_int32]),
+    "xted_tree_stats": (ctypes.c_int, [_I32P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
+                                       _I32P, _I32P]),
+}
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("plan_ms", ctypes.c_double), ("dp_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("cells", ctypes.c_int64),
+                ("tables", ctypes.c_int64), ("levels", ctypes.c_int32),
+                ("launches", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("status", ctypes.c_int32), ("dist", ctypes.c_int32),
+                ("latency_ms", ctypes.c_double), ("queue_ms", ctypes.c_double)]
+
+
+class XtedError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"xted status {status}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing; build it with `make -C x-ted_amd`")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_I32P)
+
+
+def _check(status):
+    if status != OK:
+        raise XtedError(status, lib().xted_last_error().decode(errors="replace"))
+
+
+def _tree(parent, label):
+    parent = np.ascontiguousarray(parent, dtype=np.int32)
+    label = np.ascontiguousarray(label, dtype=np.int32)
+    if parent.shape != label.shape or parent.ndim != 1:
+        raise ValueError("parent and label must be 1-D arrays of equal length")
+    return parent, label
+
+
+# ---- host utilities ---------------------------------------------------------
+
+def derive_seed(config, index):
+    return int(lib().xted_derive_seed(config, index))
+
+
+def generate(kind, n, alphabet, seed, depth_min=0, depth_max=0):
+    """Seeded preorder tree: (parent, label) int32 arrays."""
+    parent = np.empty(n, np.int32)
+    label = np.empty(n, np.int32)
+    _check(lib().xted_generate(kind, n, alphabet, seed, depth_min, depth_max, _p(parent),
+                               _p(label)))
+    return parent, label
+
+
+def validate(parent):
+    parent = np.ascontiguousarray(parent, dtype=np.int32)
+    return lib().xted_validate(_p(parent), len(parent))
+
+
+def tree_stats(parent):
+    """(S = sum of keyroot subtree sizes, depth, number of keyroots)."""
+    parent = np.ascontiguousarray(parent, dtype=np.int32)
+    ksum = ctypes.c_int64()
+    depth = ctypes.c_int32()
+    nk = ctypes.c_int32()
+    _check(lib().xted_tree_stats(_p(parent), len(parent), ctypes.byref(ksum), ctypes.byref(depth),
+                                 ctypes.byref(nk)))
+    return ksum.value, depth.value, nk.value
+
+
+# ---- device context ----------------------------------------------------------
+
+class Context:
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        _check(lib().xted_create(device, ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().xted_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def ted(self, x, y, want_td=True):
+        """x, y = (parent, label). Returns (dist, td or None, Stats)."""
+        xp, xl = _tree(*x)
+        yp, yl = _tree(*y)
+        m, n = len(xp), len(yp)
+        td = np.empty((m, n), np.int32) if (want_td and m and n) else None
+        dist = ctypes.c_int32()
+        st = Stats()
+        _check(lib().xted_ted(self._h, _p(xp), _p(xl), m, _p(yp), _p(yl), n,
+                              _p(td) if td is not None else None, ctypes.byref(dist),
+                              ctypes.byref(st)))
+        return dist.value, td, st
+
+    def tree_structure(self, parent):
+        """Device-computed (orl, keyroots descending, keyroot depths)."""
+        parent = np.ascontiguousarray(parent, dtype=np.int32)
+        n = len(parent)
+        orl = np.empty(n, np.int32)
+        kr = np.empty(n, np.int32)
+        kd = np.empty(n, np.int32)
+        K = ctypes.c_int32()
+        _check(lib().xted_tree_structure(self._h, _p(parent), n, _p(orl), _p(kr), ctypes.byref(K),
+                                         _p(kd)))
+        return orl, kr[:K.value].copy(), kd[:K.value].copy()
+
+    def pool_add(self, trees):
+        """trees: list of (parent, label). Returns the pool ids."""
+        if not trees:
+            return np.empty(0, np.int32)
+        sizes = np.array([len(t[0]) for t in trees], np.int32)
+        parents = np.ascontiguousarray(np.concatenate([np.asarray(t[0], np.int32) for t in trees]))
+        labels = np.ascontiguousarray(np.concatenate([np.asarray(t[1], np.int32) for t in trees]))
+        ids = np.empty(len(trees), np.int32)
+        _check(lib().xted_pool_add(self._h, len(trees), _p(sizes), _p(parents), _p(labels),
+                                   _p(ids)))
+        return ids
+
+    def pool_clear(self):
+        _check(lib().xted_pool_clear(self._h))
+
+    def pool_size(self):
+        n = ctypes.c_int32()
+        _check(lib().xted_pool_size(self._h, ctypes.byref(n)))
+        return n.value
+
+    def batch(self, xs, ys, td_shapes=None):
+        """Distances of pool pairs; td_shapes (list of (m, n)) also returns the tables."""
+        xs = np.ascontiguousarray(xs, dtype=np.int32)
+        ys = np.ascontiguousarray(ys, dtype=np.int32)
+        dist = np.empty(len(xs), np.int32)
+        st = Stats()
+        tds = None
+        arr = None
+        if td_shapes is not None:
+            tds = [np.empty(s, np.int32) for s in td_shapes]
+            arr = (ctypes.c_void_p * len(tds))(*[t.ctypes.data for t in tds])
+        _check(lib().xted_batch(self._h, len(xs), _p(xs), _p(ys), _p(dist),
+                                ctypes.cast(arr, ctypes.c_void_p) if arr is not None else None,
+                                ctypes.byref(st)))
+        return dist, tds, st
+
+    def set_tuning(self, lds_per_wave, min_waves=1, wide_cols=1 << 30):
+        _check(lib().xted_set_tuning(self._h, lds_per_wave, min_waves, wide_cols))
+
+    def set_profile(self, on=True):
+        _check(lib().xted_set_profile(self._h, 1 if on else 0))
+
+    def class_profile(self):
+        """{class: (launches, tables, cells, ms)} for the classes the last call used."""
+        out = {}
+        for c in range(NUM_CLASSES):
+            la, ta, ce = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+            ms = ctypes.c_double()
+            _check(lib().xted_class_profile(self._h, c, ctypes.byref(la), ctypes.byref(ta),
+                                            ctypes.byref(ce), ctypes.byref(ms)))
+            if la.value:
+                out[c] = (la.value, ta.value, ce.value, ms.value)
+        return out
+
+    def force_class(self, cls=CLASS_AUTO, lane_enable=True):
+        _check(lib().xted_force_class(self._h, cls, 1 if lane_enable else 0))
+
+
+class Server:
+    """Continuous-batching server over a context's pool."""
+
+    def __init__(self, ctx, hbm_budget, max_batch=4096):
+        self._h = ctypes.c_void_p()
+        self.ctx = ctx
+        _check(lib().xted_server_create(ctx._h, hbm_budget, max_batch, ctypes.byref(self._h)))
+
+    def submit(self, x_id, y_id, tag):
+        _check(lib().xted_server_submit(self._h, x_id, y_id, tag))
+
+    def poll(self, cap=1024, timeout_ms=0):
+        buf = (Result * cap)()
+        count = ctypes.c_int32()
+        _check(lib().xted_server_poll(self._h, buf, cap, timeout_ms, ctypes.byref(count)))
+        return [(r.tag, r.status, r.dist, r.latency_ms, r.queue_ms) for r in buf[:count.value]]
+
+    def close(self):
+        if self._h:
+            lib().xted_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
