@@ -97,9 +97,10 @@ def main():
     ids = ctx.pool_add(trees)
     xs, ys = ids[0::2].copy(), ids[1::2].copy()
 
+    dist = None
     for _ in range(args.warmup):
         dist, _, st = ctx.batch(xs, ys)
-    log(f"warmup done: {st.tables} tables, {st.cells} cells, {st.levels} levels")
+        log(f"warmup done: {st.tables} tables, {st.cells} cells, {st.levels} levels")
 
     ctx.set_profile(True)
     times = []
@@ -107,6 +108,8 @@ def main():
         t = time.perf_counter()
         d, _, st = ctx.batch(xs, ys)
         times.append(time.perf_counter() - t)
+        if dist is None:
+            dist = d
         if not np.array_equal(d, dist):
             raise SystemExit("batched distances changed between steps")
         log(f"step {k}: {times[-1] * 1e3:.1f} ms (plan {st.plan_ms:.1f} ms, dp {st.dp_ms:.1f} ms)")

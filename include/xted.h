@@ -100,10 +100,11 @@ int xted_batch(xted_context* ctx, int32_t npairs, const int32_t* x_ids, const in
 /* ---- tuning and test hooks ------------------------------------------------------
  * Granularity cut-overs, the role of the fixed thresholds 15 / 18,750 /
  * 25,250,000 in X-TED_GPU/parallel_X_TED.cu:382. lds_per_wave: slot bytes
- * one wave may own before a table gets more waves; tables with at least
- * wide_cols columns get at least min_waves waves. */
-int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t min_waves,
-                    int32_t wide_cols);
+ * one wave may own before a table gets more waves; max_cpl (1, 2, 4, 8 or
+ * 16): columns per lane before a table gets more waves; tables with at
+ * least wide_cols columns get at least min_waves waves. */
+int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t max_cpl,
+                    int32_t min_waves, int32_t wide_cols);
 /* Forces every table that fits into class cls (XTED_CLASS_AUTO restores the
  * rule); lane_enable = 0 sends single-row tables to the row-scan kernels. */
 int xted_force_class(xted_context* ctx, int32_t cls, int32_t lane_enable);

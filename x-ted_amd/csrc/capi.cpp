@@ -188,13 +188,16 @@ int xted_batch(xted_context* ctx, int32_t npairs, const int32_t* x_ids, const in
   });
 }
 
-int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t min_waves,
-                    int32_t wide_cols) {
-  if (!ctx || lds_per_wave < 0 || min_waves < 1 || min_waves > kMaxRowWaves || wide_cols < 0) {
+int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t max_cpl,
+                    int32_t min_waves, int32_t wide_cols) {
+  const bool pow2 = max_cpl == 1 || max_cpl == 2 || max_cpl == 4 || max_cpl == 8 || max_cpl == 16;
+  if (!ctx || lds_per_wave < 0 || !pow2 || min_waves < 1 || min_waves > kMaxRowWaves ||
+      wide_cols < 0) {
     set_error("xted_set_tuning: invalid argument");
     return ERR_INVALID_ARG;
   }
   ctx->ctx.rule.lds_per_wave = lds_per_wave;
+  ctx->ctx.rule.max_cpl = max_cpl;
   ctx->ctx.rule.min_waves = min_waves;
   ctx->ctx.rule.wide_cols = wide_cols;
   return OK;

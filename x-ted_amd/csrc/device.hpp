@@ -6,6 +6,12 @@
 //   leaf rank      lrank                      [node_off + i], i in [0, n]
 //   keyroot arrays kr/krkd/krsize/krleaf      [node_off + k], k in [0, K)
 //                  (keyroots in descending preorder; K <= n)
+//   sorted keyroots skr/skd/ssize/sleaf       [node_off + k], k in [0, K):
+//                  the K1 leaf keyroots first, then the others by ascending
+//                  nesting height d, those of height d at
+//                  [K1 + kdoff[d], K1 + kdoff[d+1])
+//   spre           [node_off + k], k in [0, K]: prefix sums of ssize
+//   kdoff          [node_off + d], d in [0, kdmax+1]
 // A pair owns a tree-distance table TD (m x n, row-major, preorder) and its
 // transpose TDT (n x m); both live in one int32 arena at td_off / tdt_off.
 #pragma once
@@ -25,6 +31,8 @@ struct TreeMeta {
   int32_t K;         // keyroot count
   int32_t kdmax;     // keyroot nesting height of the root
   int32_t nleaves;   // leaf count
+  int32_t K1;        // keyroots that are leaves
+  int32_t pad;
 };
 
 struct PairMeta {
@@ -45,6 +53,12 @@ struct PoolView {
   const int32_t* krkd;   // nesting height per keyroot
   const int32_t* krsize; // subtree size per keyroot
   const int32_t* krleaf; // leaves in subtree per keyroot
+  const int32_t* skr;    // sorted keyroots (leaf keyroots, then by nesting height)
+  const int32_t* skd;
+  const int32_t* ssize;
+  const int32_t* sleaf;
+  const int32_t* spre;   // prefix sums of ssize (K+1 entries)
+  const int32_t* kdoff;
   const TreeMeta* trees;
 };
 
@@ -83,6 +97,7 @@ struct ClassRule {
   int32_t min_waves;     // lower bound on W for tables with >= wide_cols columns
   int32_t wide_cols;
   int32_t lds_max;       // slot bytes one workgroup may hold (device limit)
+  int32_t max_cpl;       // most columns per lane before a table gets more waves
 };
 
 XTED_HD inline int ilog2_ceil(int v) {
@@ -99,7 +114,7 @@ XTED_HD inline int table_class(int a, int b, int64_t lds_need, const ClassRule& 
   if (cols > kMaxCols || lds_need > rule.lds_max) return kClassXL;
   int wi = 0;
   while (wi < 4 && lds_need > (int64_t)rule.lds_per_wave * (1 << wi)) ++wi;
-  while (wi < 4 && cols > 64 * 16 * (1 << wi)) ++wi;
+  while (wi < 4 && cols > 64 * rule.max_cpl * (1 << wi)) ++wi;
   if (cols >= rule.wide_cols) {
     while (wi < 4 && (1 << wi) < rule.min_waves) ++wi;
   }

@@ -78,7 +78,8 @@ Context::~Context() {
   }
   for (DevBuf* b : {&parent_, &label_, &orl_, &kdep_, &kdn_, &lrank_, &kr_, &krkd_, &krsize_,
                     &krleaf_, &trees_, &prep_scratch_, &prep_off_, &pairs_, &blk_, &counts_,
-                    &cells_, &cursor_, &maxlds_, &tasks_, &td_, &dist_})
+                    &cells_, &cursor_, &maxlds_, &tasks_, &td_, &dist_, &skr_, &skd_, &ssize_,
+                    &sleaf_, &spre_, &kdoff_, &lane_off_, &lane_tot_, &lane_cells_})
     b->release();
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
@@ -123,6 +124,12 @@ PoolView Context::view() const {
   v.krkd = krkd_.as<int32_t>();
   v.krsize = krsize_.as<int32_t>();
   v.krleaf = krleaf_.as<int32_t>();
+  v.skr = skr_.as<int32_t>();
+  v.skd = skd_.as<int32_t>();
+  v.ssize = ssize_.as<int32_t>();
+  v.sleaf = sleaf_.as<int32_t>();
+  v.spre = spre_.as<int32_t>();
+  v.kdoff = kdoff_.as<int32_t>();
   v.trees = trees_.as<TreeMeta>();
   return v;
 }
@@ -130,7 +137,7 @@ PoolView Context::view() const {
 int Context::grow_pool(int64_t nodes) {
   const size_t bytes = (size_t)nodes * sizeof(int32_t);
   for (DevBuf* b : {&parent_, &label_, &orl_, &kdep_, &kdn_, &lrank_, &kr_, &krkd_, &krsize_,
-                    &krleaf_})
+                    &krleaf_, &skr_, &skd_, &ssize_, &sleaf_, &spre_, &kdoff_})
     ST_TRY(b->grow(bytes, true, stream_));
   return OK;
 }
@@ -185,7 +192,7 @@ int Context::pool_add(int32_t ntrees, const int32_t* sizes, const int32_t* paren
     const int64_t o = offs[t] - pool_nodes_;
     std::memcpy(&hp[o], parents + in_off, sizeof(int32_t) * n);
     std::memcpy(&hl[o], labels + in_off, sizeof(int32_t) * n);
-    tm[t] = TreeMeta{offs[t], n, 0, 0, 0};
+    tm[t] = TreeMeta{offs[t], n, 0, 0, 0, 0, 0};
     // Host-side cell count for reporting only.
     std::vector<int32_t> orl(n);
     compute_orl(parents + in_off, n, orl.data());
@@ -217,6 +224,12 @@ int Context::pool_add(int32_t ntrees, const int32_t* sizes, const int32_t* paren
   pa.krkd = krkd_.as<int32_t>();
   pa.krsize = krsize_.as<int32_t>();
   pa.krleaf = krleaf_.as<int32_t>();
+  pa.skr = skr_.as<int32_t>();
+  pa.skd = skd_.as<int32_t>();
+  pa.ssize = ssize_.as<int32_t>();
+  pa.sleaf = sleaf_.as<int32_t>();
+  pa.spre = spre_.as<int32_t>();
+  pa.kdoff = kdoff_.as<int32_t>();
   pa.trees = trees_.as<TreeMeta>();
   pa.scratch = prep_scratch_.as<int32_t>();
   pa.scratch_off = prep_off_.as<int64_t>();
@@ -277,7 +290,7 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   HIP_TRY(hipSetDevice(device_));
   std::vector<PairMeta> pm(npairs);
   std::vector<int64_t> blk(npairs + 1, 0);
-  int64_t td_total = 0;
+  int64_t td_total = 0, planned = 0;
   int32_t nlevels = 0;
   for (int32_t i = 0; i < npairs; ++i) {
     const int32_t x = xs[i], y = ys[i];
@@ -287,9 +300,12 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     }
     const TreeMeta &a = meta_[x], &b = meta_[y];
     const int64_t mn = (int64_t)a.n * b.n;
-    pm[i] = PairMeta{x, y, td_total, td_total + mn};
-    td_total += 2 * mn;
-    blk[i + 1] = blk[i] + plan_blocks(a.K, b.K, plan_chunk);
+    pm[i] = PairMeta{x, y, td_total + kTdPad, td_total + 2 * kTdPad + mn};
+    td_total += 2 * mn + 3 * kTdPad;
+    // With the lane kernel on, the plan covers only non-leaf keyroot pairs.
+    const int32_t ka = lane_enable ? a.K - a.K1 : a.K, kb = lane_enable ? b.K - b.K1 : b.K;
+    blk[i + 1] = blk[i] + plan_blocks(ka, kb, plan_chunk);
+    planned += (int64_t)ka * kb;
     nlevels = std::max(nlevels, a.kdmax + b.kdmax + 1);
     st.tables += (int64_t)a.K * b.K;
     st.cells += info_[x].ksum * info_[y].ksum;
@@ -301,9 +317,12 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   ST_TRY(cells_.grow(sizeof(unsigned long long) * nb, false, stream_));
   ST_TRY(cursor_.grow(sizeof(unsigned long long) * nb, false, stream_));
   ST_TRY(maxlds_.grow(sizeof(int32_t) * kNumClasses, false, stream_));
-  ST_TRY(tasks_.grow(sizeof(uint64_t) * st.tables, false, stream_));
+  ST_TRY(tasks_.grow(sizeof(uint64_t) * std::max<int64_t>(planned, 1), false, stream_));
   ST_TRY(td_.grow(sizeof(int32_t) * td_total, false, stream_));
   ST_TRY(dist_.grow(sizeof(int32_t) * npairs, false, stream_));
+  ST_TRY(lane_off_.grow(sizeof(unsigned long long) * nlevels * (npairs + 1), false, stream_));
+  ST_TRY(lane_tot_.grow(sizeof(unsigned long long) * nlevels, false, stream_));
+  ST_TRY(lane_cells_.grow(sizeof(unsigned long long) * nlevels, false, stream_));
   HIP_TRY(hipMemcpyAsync(pairs_.p, pm.data(), sizeof(PairMeta) * npairs, hipMemcpyHostToDevice,
                          stream_));
   HIP_TRY(hipMemcpyAsync(blk_.p, blk.data(), sizeof(int64_t) * (npairs + 1),
@@ -327,6 +346,25 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   pa.tasks = tasks_.as<uint64_t>();
   launch_plan(pa, blk[npairs], stream_);
   HIP_TRY(hipGetLastError());
+  LaneArgs la;
+  la.pool = pa.pool;
+  la.pairs = pa.pairs;
+  la.npairs = npairs;
+  la.level = 0;
+  la.offsets = lane_off_.as<unsigned long long>();
+  la.totals = lane_tot_.as<unsigned long long>();
+  la.cells = lane_cells_.as<unsigned long long>();
+  la.td = td_.as<int32_t>();
+  h_lane_tot_.assign(nlevels, 0);
+  h_lane_cells_.assign(nlevels, 0);
+  if (lane_enable) {
+    launch_lane_offsets(la, nlevels, stream_);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(h_lane_tot_.data(), lane_tot_.p, sizeof(unsigned long long) * nlevels,
+                           hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipMemcpyAsync(h_lane_cells_.data(), lane_cells_.p,
+                           sizeof(unsigned long long) * nlevels, hipMemcpyDeviceToHost, stream_));
+  }
   h_counts_.assign(nb, 0);
   int32_t h_maxlds[kNumClasses] = {0};
   HIP_TRY(hipMemcpyAsync(h_counts_.data(), counts_.p, sizeof(unsigned long long) * nb,
@@ -351,10 +389,11 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   da.pairs = pa.pairs;
   da.td = td_.as<int32_t>();
   // With profiling on, an event pair brackets every launch (same stream).
-  std::vector<std::pair<int, int>> prof;  // (class, bucket) per timed launch
+  std::vector<std::pair<int, int>> prof;  // (class, bucket or -1-level) per timed launch
   if (profile) {
     size_t need = 0;
     for (unsigned long long c : h_counts_) need += c ? 2 : 0;
+    for (unsigned long long c : h_lane_tot_) need += c ? 2 : 0;
     while (prof_ev_.size() < need) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
@@ -363,12 +402,22 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   }
   unsigned long long off = 0;
   for (int L = 0; L < nlevels; ++L) {
+    if (h_lane_tot_[L]) {
+      la.level = L;
+      if (profile) HIP_TRY(hipEventRecord(prof_ev_[2 * prof.size()], stream_));
+      launch_lane_level(la, h_lane_tot_[L], stream_);
+      if (profile) {
+        HIP_TRY(hipEventRecord(prof_ev_[2 * prof.size() + 1], stream_));
+        prof.emplace_back(kClassLane, -1 - L);
+      }
+      ++st.launches;
+    }
     for (int c = 0; c < kNumClasses; ++c) {
       const unsigned long long cnt = h_counts_[(size_t)L * kNumClasses + c];
       if (cnt) {
         da.tasks = tasks_.as<uint64_t>() + off;
         da.ntasks = (int64_t)cnt;
-        const int lds = (h_maxlds[c] + 15) & ~15;
+        const int lds = std::max(64, (h_maxlds[c] + 15) & ~15);
         if (profile) HIP_TRY(hipEventRecord(prof_ev_[2 * prof.size()], stream_));
         if (!launch_forest(c, da, lds, stream_)) {
           set_error("run_batch: no kernel for class " + std::to_string(c));
@@ -405,8 +454,9 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
       ClassProfile& cp = class_profile[prof[i].first];
       cp.launches += 1;
       cp.ms += ms;
-      cp.tables += (int64_t)h_counts_[prof[i].second];
-      cp.cells += (int64_t)h_cells_[prof[i].second];
+      const int b = prof[i].second;
+      cp.tables += (int64_t)(b >= 0 ? h_counts_[b] : h_lane_tot_[-1 - b]);
+      cp.cells += (int64_t)(b >= 0 ? h_cells_[b] : h_lane_cells_[-1 - b]);
     }
   }
   float plan_ms = 0, dp_ms = 0;

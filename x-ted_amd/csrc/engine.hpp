@@ -75,7 +75,7 @@ class Context {
                 int32_t* const* td_out, BatchStats* stats);
 
   // Tuning / test hooks -----------------------------------------------------
-  ClassRule rule{kLdsPerWave, 1, 1 << 30, kLdsMaxTable};
+  ClassRule rule{kLdsPerWave, 1, 1 << 30, kLdsMaxTable, 8};
   int32_t lds_limit = 0;  // device's dynamic LDS per workgroup for the slots
   int32_t force_class = -1;
   int32_t lane_enable = 1;
@@ -96,10 +96,12 @@ class Context {
   std::vector<HostTreeInfo> info_;
   int64_t pool_nodes_ = 0;  // used entries (sum of n+1)
   DevBuf parent_, label_, orl_, kdep_, kdn_, lrank_, kr_, krkd_, krsize_, krleaf_, trees_;
+  DevBuf skr_, skd_, ssize_, sleaf_, spre_, kdoff_;
   DevBuf prep_scratch_, prep_off_;
   // batch
   DevBuf pairs_, blk_, counts_, cells_, cursor_, maxlds_, tasks_, td_, dist_;
-  std::vector<unsigned long long> h_counts_, h_cells_;
+  DevBuf lane_off_, lane_tot_, lane_cells_;
+  std::vector<unsigned long long> h_counts_, h_cells_, h_lane_tot_, h_lane_cells_;
   std::vector<hipEvent_t> prof_ev_;
   int grow_pool(int64_t nodes);
   PoolView view() const;

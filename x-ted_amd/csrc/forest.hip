@@ -62,42 +62,58 @@ __device__ __forceinline__ int wave_shr1(int v, int fill) {
 // Tasks with the same column keyroot sit next to each other, so the orl and
 // label loads of one wave are mostly uniform.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void lane_table_kernel(DPArgs A) {
-  const int64_t tix = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (tix >= A.ntasks) return;
-  const uint64_t task = A.tasks[tix];
-  const PairMeta pm = A.pairs[task_pair(task)];
-  const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
-  const int32_t i0 = (int32_t)task_i0(task), j0 = (int32_t)task_j0(task);
-  const int32_t* orlx = A.pool.orl + tx.node_off;
-  const int32_t* orly = A.pool.orl + ty.node_off;
+__device__ __forceinline__ void lane_table(const PoolView& P, const PairMeta& pm, int32_t i0,
+                                           int32_t j0, int32_t* td) {
+  const TreeMeta tx = P.trees[pm.tx], ty = P.trees[pm.ty];
+  const int32_t* orlx = P.orl + tx.node_off;
+  const int32_t* orly = P.orl + ty.node_off;
   const int32_t a = orlx[i0] - i0 + 1, b = orly[j0] - j0 + 1;
   const bool rows_x = a <= b;
   const int64_t offR = rows_x ? tx.node_off : ty.node_off;
   const int64_t offC = rows_x ? ty.node_off : tx.node_off;
-  const int32_t* labR = A.pool.label + offR;
-  const int32_t* orlC = A.pool.orl + offC;
-  const int32_t* labC = A.pool.label + offC;
+  const int32_t* labR = P.label + offR;
+  const int32_t* orlC = P.orl + offC;
+  const int32_t* labC = P.label + offC;
   const int32_t r0 = rows_x ? i0 : j0, c0 = rows_x ? j0 : i0;
   // TD in [r][c] and [c][r] layouts for this orientation.
-  int32_t* TDrc = A.td + (rows_x ? pm.td_off : pm.tdt_off);
-  int32_t* TDcr = A.td + (rows_x ? pm.tdt_off : pm.td_off);
+  int32_t* TDrc = td + (rows_x ? pm.td_off : pm.tdt_off);
+  int32_t* TDcr = td + (rows_x ? pm.tdt_off : pm.td_off);
   const int64_t ldrc = rows_x ? ty.n : tx.n;
   const int64_t ldcr = rows_x ? tx.n : ty.n;
   const int32_t cmax = orlC[c0] + 1;
   const int32_t orl_c0 = cmax - 1;
   const int32_t labr = labR[r0];
+  // Loads run kLaneAhead columns ahead of the recurrence: slot k of the
+  // rings holds column c-k's orl, label and TD value. The loads are
+  // unconditional (past c0 they re-read column c0) and their values are
+  // only combined when used, so no load waits inside a branch.
+  constexpr int kLaneAhead = 4;
+  int32_t ring_o[kLaneAhead], ring_l[kLaneAhead], ring_v[kLaneAhead];
+  auto fetch = [&](int32_t c, int k) {
+    ring_o[k] = orlC[c];
+    ring_l[k] = labC[c];
+    ring_v[k] = TDcr[(int64_t)c * ldcr + r0];
+  };
+#pragma unroll
+  for (int k = 0; k < kLaneAhead; ++k) fetch(max(cmax - 1 - k, c0), k);
   int32_t g = 1;  // F[r0][cmax] = imax - r0
+  // Unrolled by the ring depth so the ring rotation is register renaming,
+  // not copies of loads still in flight.
+#pragma unroll 4
   for (int32_t c = cmax - 1; c >= c0; --c) {
     const int32_t t = cmax - 1 - c;
-    const int32_t oc = orlC[c];
-    int32_t d;
-    const bool tree = (oc == orl_c0);
-    if (tree) {
-      d = t + (labr != labC[c] ? 1 : 0);
-    } else {
-      d = (cmax - oc - 1) + TDcr[(int64_t)c * ldcr + r0];
+    const int32_t oc = ring_o[0], lab = ring_l[0], v = ring_v[0];
+#pragma unroll
+    for (int k = 0; k + 1 < kLaneAhead; ++k) {
+      ring_o[k] = ring_o[k + 1];
+      ring_l[k] = ring_l[k + 1];
+      ring_v[k] = ring_v[k + 1];
     }
+    fetch(max(c - kLaneAhead, c0), kLaneAhead - 1);
+    const bool tree = (oc == orl_c0);
+    // tree cell: F[imax][c+1] + rename = t + (lab differ); forest cell:
+    // F[imax][orl(c)+1] + TD[r0][c] = (cmax - orl(c) - 1) + TD
+    const int32_t d = tree ? t + (labr != lab ? 1 : 0) : (cmax - oc - 1) + v;
     const int32_t av = min(d, t + 2);
     g = min(av, g + 1);
     if (tree) {
@@ -107,12 +123,63 @@ __global__ __launch_bounds__(256) void lane_table_kernel(DPArgs A) {
   }
 }
 
+// Single-row tables of one level, without a task list: thread g finds its
+// pair in the level's prefix over pairs, then its cell of rectangle A
+// (consecutive lanes share the y keyroot) or B (consecutive lanes share the
+// x keyroot).
+__global__ __launch_bounds__(256) void lane_level_kernel(LaneArgs A, unsigned long long total) {
+  const unsigned long long g = (unsigned long long)blockIdx.x * 256 + threadIdx.x;
+  if (g >= total) return;
+  const unsigned long long* off = A.offsets + (int64_t)A.level * (A.npairs + 1);
+  int lo = 0, hi = A.npairs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const int p = lo;
+  int32_t sz[5];
+  (void)lane_rect_count(A, p, A.level, sz);
+  const PairMeta pm = A.pairs[p];
+  const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
+  int32_t q = (int32_t)(g - off[p]);
+  int32_t k, l;
+  if (q < sz[0]) {
+    k = q % tx.K1;
+    l = sz[3] + q / tx.K1;
+  } else {
+    q -= sz[0];
+    l = q % ty.K1;
+    k = sz[4] + q / ty.K1;
+  }
+  lane_table(A.pool, pm, A.pool.skr[tx.node_off + k], A.pool.skr[ty.node_off + l], A.td);
+}
+
+void launch_lane_level(const LaneArgs& a, unsigned long long total, hipStream_t s) {
+  if (total == 0) return;
+  hipLaunchKernelGGL(lane_level_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a,
+                     total);
+}
+
 // ---------------------------------------------------------------------------
-// Workgroup row scan: one table per workgroup of W waves; lane l of wave w
-// owns columns t in [(w*64+l)*C, (w*64+l)*C + C) (t counted from jmax).
+// Workgroup row scan: one table per workgroup of W waves, C columns per
+// lane. Columns are counted from jmax: t = jmax-1-j. Wave w owns the block
+// t in [w*64*C, (w+1)*64*C). Inside it, lanes own E contiguous columns per
+// group and the G = C/E groups are interleaved: lane l's element (g, e) is
+//   t = w*64*C + g*64*E + l*E + e,   E = min(C, 4).
+// So one group of a TD row is a 64*E-int contiguous span that the wave
+// reads with one coalesced E-int load per lane, and the row scan is G
+// chained wave scans over groups.
 // ---------------------------------------------------------------------------
+typedef int32_t int4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef int32_t int2u __attribute__((ext_vector_type(2), aligned(4)));
+
 template <int W, int C>
 __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
+  constexpr int E = C <= 4 ? C : 4;               // contiguous columns per group
+  constexpr int G = C / E;                        // groups per lane
+  constexpr int S = C <= 8 ? 4 : 2;  // TD ring slots (divides 64)
+  constexpr int PD = S - 1;          // rows of TD prefetched
   extern __shared__ uint16_t slots[];
   __shared__ int32_t xtot[W];  // per-wave scan totals
   __shared__ int32_t xbnd[W];  // per-wave last-column F value of the row
@@ -143,128 +210,202 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   const int32_t rmax = orlR[r0] + 1, cmax = orlC[c0] + 1;
   const int32_t ncols = cmax - c0;
   const int32_t base = kdepR[r0];
-  const int32_t orl_r0 = rmax - 1, orl_c0 = cmax - 1;
+  const int32_t orl_c0 = cmax - 1;
   const int32_t lr_c0 = lrankC[c0];
   const int32_t width = lrankC[cmax] - lr_c0 - 1;
 
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int32_t tbase = (wave * 64 + lane) * C;
+  const int32_t wbase = wave * 64 * C;
+#define TCOL(g, e) (wbase + (g) * 64 * E + lane * E + (e))
 
-  // Per-column state, in registers. Column e of this lane is t = tbase + e,
-  // c = cmax-1-t; it is valid while t < ncols. fcol = orl_C(c)+1, so the
-  // column is a tree column exactly when fcol == cmax. fw packs the compact
-  // far index (high 16 bits) and 1 + the saved-row write index (low 16 bits,
-  // 0 = not a leaf+1 column). Fp holds F[r+1][c]; F[r+1][c+1] is Fp[e-1],
-  // or fpr0 for e = 0.
-  int32_t labc[C], fcol[C], fw[C], Fp[C], tdc[C], tdn[C];
+  // Per-column state, element j = g*E + e. fcol = orl_C(c)+1, so the column
+  // is a tree column exactly when fcol == cmax. fw packs the compact far
+  // index (high 16 bits) and 1 + the saved-row write index (low 16 bits, 0 =
+  // not a leaf+1 column). Fp holds F[r+1][c]; F[r+1][c+1] is Fp of element
+  // j-1, or fpr0[g] for e = 0.
+  int32_t labc[C], fcol[C], fw[C], Fp[C], tdq[S][C];
+  int32_t fpr0[G];
 #pragma unroll
-  for (int e = 0; e < C; ++e) {
-    const int32_t t = tbase + e;
-    const int32_t c = cmax - 1 - t;
-    labc[e] = 0;
-    fcol[e] = cmax;
-    fw[e] = 0;
-    if (t < ncols) {
-      const int32_t oc = orlC[c];
-      labc[e] = labC[c];
-      fcol[e] = oc + 1;
-      const int32_t fidx = (oc == orl_c0) ? 0 : lrankC[oc] - lr_c0;
-      const int32_t widx1 = (c > c0 && orlC[c - 1] == c - 1) ? lrankC[c - 1] - lr_c0 + 1 : 0;
-      fw[e] = (fidx << 16) | widx1;
+  for (int g = 0; g < G; ++g) {
+    fpr0[g] = TCOL(g, 0);  // F[rmax][c+1] = cmax-1-c = t
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = g * E + e;
+      const int32_t t = TCOL(g, e);
+      const int32_t c = cmax - 1 - t;
+      labc[j] = 0;
+      fcol[j] = cmax;
+      fw[j] = 0;
+      if (t < ncols) {
+        const int32_t oc = orlC[c];
+        labc[j] = labC[c];
+        fcol[j] = oc + 1;
+        const int32_t fidx = (oc == orl_c0) ? 0 : lrankC[oc] - lr_c0;
+        const int32_t widx1 = (c > c0 && orlC[c - 1] == c - 1) ? lrankC[c - 1] - lr_c0 + 1 : 0;
+        fw[j] = (fidx << 16) | widx1;
+      }
+      Fp[j] = t + 1;  // F[rmax][c] = cmax - c
     }
-    Fp[e] = t + 1;  // F[rmax][c] = cmax - c
-    tdc[e] = 0;
-    tdn[e] = 0;
-  }
-  int32_t fpr0 = tbase;  // F[rmax][c+1] for e = 0
-  {
-    const int32_t* row = TDo + (int64_t)(rmax - 1) * ldo;
-#pragma unroll
-    for (int e = 0; e < C; ++e)
-      if (tbase + e < ncols) tdc[e] = row[cmax - 1 - (tbase + e)];
   }
 
-  for (int32_t r = rmax - 1; r >= r0; --r) {
-    // Prefetch the next row's TD values.
-    if (r > r0) {
-      const int32_t* row = TDo + (int64_t)(r - 1) * ldo;
+  // TD row rr of this lane's columns into out[]. Loads are unconditional, so
+  // the compiler never waits for them inside a branch: a group with no valid
+  // column reads the row's first E ints instead, a partly valid group reads
+  // up to E-1 ints before column c0 (the arena keeps kTdPad ints of padding
+  // before and after every table, so both stay inside the allocation), and
+  // invalid elements are masked afterwards.
+  auto load_row = [&](int32_t rr, int32_t* out) {
+    const int32_t* row = TDo + (int64_t)rr * ldo;
 #pragma unroll
-      for (int e = 0; e < C; ++e)
-        if (tbase + e < ncols) tdn[e] = row[cmax - 1 - (tbase + e)];
-    }
-    const int32_t orr = orlR[r];
-    const int32_t labr = labR[r];
-    const bool tree_row = (orr == orl_r0);
-    const int32_t Rf = orr + 1;
-    const int32_t fslot = (Rf < rmax) ? kdepR[r] - base - 1 : 0;
-    const bool wrow = (r > r0) && (orlR[r - 1] == r - 1);
-    const int32_t wslot = wrow ? kdepR[r - 1] - base - 1 : 0;
-    const uint16_t* frow = slots + fslot * width;
-
-    int32_t h[C];
-#pragma unroll
-    for (int e = 0; e < C; ++e) {
-      const int32_t t = tbase + e;
-      const bool tree_col = (fcol[e] == cmax);
-      int32_t d;
-      if (tree_row && tree_col) {
-        const int32_t fpr = e ? Fp[e - 1] : fpr0;
-        d = fpr + (labr != labc[e] ? 1 : 0);
+    for (int g = 0; g < G; ++g) {
+      const int32_t tb = TCOL(g, 0);
+      const int32_t off = tb < ncols ? cmax - E - tb : c0;
+      if constexpr (E == 4) {
+        const int4u v = *reinterpret_cast<const int4u*>(row + off);
+        out[g * 4 + 0] = v.w;
+        out[g * 4 + 1] = v.z;
+        out[g * 4 + 2] = v.y;
+        out[g * 4 + 3] = v.x;
+      } else if constexpr (E == 2) {
+        const int2u v = *reinterpret_cast<const int2u*>(row + off);
+        out[g * 2 + 0] = v.y;
+        out[g * 2 + 1] = v.x;
       } else {
-        int32_t farv;
-        if (Rf == rmax) {
-          farv = cmax - fcol[e];
-        } else if (tree_col) {
-          farv = rmax - Rf;
-        } else {
-          farv = (int32_t)frow[(uint32_t)fw[e] >> 16];
+        out[g] = row[off];
+      }
+    }
+  };
+#pragma unroll
+  for (int q = 0; q < PD; ++q) load_row(max(rmax - 1 - q, r0), tdq[q]);
+
+  // Row metadata, 64 rows per chunk: lane i holds row rmax-1-(64q+i) as
+  // (label, Rf = orl_R(r)+1, slots = fslot | wslot << 16 | wrow << 31), read
+  // per row with readlane; the next chunk is loaded one chunk ahead.
+  auto load_meta = [&](int q, int32_t& mlab, int32_t& mrf, int32_t& mslot) {
+    const int32_t rr = rmax - 1 - (64 * q + lane);
+    const int32_t r = max(rr, r0);          // rows past the table read row r0
+    const int32_t rp = max(r - 1, r0);
+    const int32_t orr = orlR[r], lab = labR[r], kd = kdepR[r];
+    const int32_t orp = orlR[rp], kdp = kdepR[rp];
+    const int32_t fs = (orr + 1 < rmax) ? kd - base - 1 : 0;
+    const int32_t ws = (r > r0 && orp == r - 1) ? (((kdp - base - 1) << 16) | (int32_t)0x80000000) : 0;
+    mlab = lab;
+    mrf = orr + 1;
+    mslot = fs | ws;
+  };
+  const int32_t nrows = rmax - r0;
+  const int32_t nchunks = (nrows + 63) >> 6;
+  int32_t mlab, mrf, mslot, nlab = 0, nrf = 0, nslot = 0;
+  load_meta(0, mlab, mrf, mslot);
+  if (nchunks > 1) load_meta(1, nlab, nrf, nslot);
+
+  for (int32_t q = 0; q < nchunks; ++q) {
+    const int32_t rows_here = min(64, nrows - 64 * q);
+    // Rows go S at a time with the inner loop fully unrolled, so row k of the
+    // table always uses ring slot k % S and every slot index is static: row
+    // r reads slot u and prefetches row r-PD into slot (u+PD) % S, the slot
+    // the previous row has just finished with.
+    for (int32_t i0r = 0; i0r < rows_here; i0r += S) {
+#pragma unroll
+    for (int u = 0; u < S; ++u) {
+      const int32_t i = i0r + u;
+      if (i >= rows_here) break;
+      const int32_t r = rmax - 1 - (64 * q + i);
+      load_row(max(r - PD, r0), tdq[(u + PD) % S]);
+
+      const int32_t labr = __builtin_amdgcn_readlane(mlab, i);
+      const int32_t Rf = __builtin_amdgcn_readlane(mrf, i);
+      const int32_t sl = __builtin_amdgcn_readlane(mslot, i);
+      const bool tree_row = (Rf == rmax);
+      const int32_t fslot = sl & 0xFFFF;
+      const bool wrow = sl < 0;
+      const int32_t wslot = (sl >> 16) & 0x7FFF;
+      const uint16_t* frow = slots + fslot * width;
+
+      // Far-row values for every element, read unconditionally (index 0 of
+      // slot 0 when unused) so the C LDS reads issue back to back.
+      int32_t sv[C];
+#pragma unroll
+      for (int j = 0; j < C; ++j) sv[j] = (int32_t)frow[(uint32_t)fw[j] >> 16];
+
+      // a[t] - t per element, local prefix per group, chained wave scans.
+      int32_t h[C], pre[G];
+      int32_t carry_w = kBig;  // min over this wave's earlier groups
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int j = g * E + e;
+          const int32_t t = TCOL(g, e);
+          const bool tree_col = (fcol[j] == cmax);
+          const int32_t fpr = e ? Fp[j - 1] : fpr0[g];
+          const int32_t dtree = fpr + (labr != labc[j] ? 1 : 0);
+          const int32_t farv = (Rf == rmax) ? cmax - fcol[j] : (tree_col ? rmax - Rf : sv[j]);
+          const int32_t d = (tree_row && tree_col) ? dtree : farv + tdq[u][j];
+          const int32_t av = min(d, Fp[j] + 1);
+          h[j] = (t < ncols) ? av - t : kBig;
+          if (e) h[j] = min(h[j], h[j - 1]);
         }
-        d = farv + tdc[e];
+        const int32_t incl = wave_incl_min(h[g * E + E - 1]);
+        pre[g] = min(carry_w, wave_shr1(incl, kBig));
+        carry_w = min(carry_w, __builtin_amdgcn_readlane(incl, 63));
       }
-      const int32_t av = min(d, Fp[e] + 1);
-      h[e] = (t < ncols) ? av - t : kBig;
-    }
-#pragma unroll
-    for (int e = 1; e < C; ++e) h[e] = min(h[e], h[e - 1]);
-    const int32_t incl = wave_incl_min(h[C - 1]);
-    int32_t carry = min((rmax - r) + 1, wave_shr1(incl, kBig));
-    if constexpr (W > 1) {
-      if (lane == 63) xtot[wave] = incl;
-      __syncthreads();
-      for (int w = 0; w < wave; ++w) carry = min(carry, xtot[w]);
-    }
-    // h becomes G = F[r][c].
-#pragma unroll
-    for (int e = 0; e < C; ++e) h[e] = tbase + e + min(carry, h[e]);
-
-    // Tree cells -> TD / TD^T; leaf+1 rows -> LDS slots.
-    uint16_t* wrow_ptr = slots + wslot * width;
-#pragma unroll
-    for (int e = 0; e < C; ++e) {
-      if (tbase + e >= ncols) continue;
-      const int32_t c = cmax - 1 - (tbase + e);
-      if (tree_row && fcol[e] == cmax) {
-        TDo[(int64_t)r * ldo + c] = h[e];
-        TDt[(int64_t)c * ldt + r] = h[e];
+      int32_t carry = (rmax - r) + 1;  // F[r][cmax] - (-1)
+      if constexpr (W > 1) {
+        if (lane == 0) xtot[wave] = carry_w;
+        __syncthreads();
+        for (int w = 0; w < wave; ++w) carry = min(carry, xtot[w]);
       }
-      const int32_t widx1 = fw[e] & 0xFFFF;
-      if (wrow && widx1) wrow_ptr[widx1 - 1] = (uint16_t)h[e];
-    }
-
-    int32_t left_in = rmax - r;  // F[r][cmax]
-    if constexpr (W > 1) {
-      if (lane == 63) xbnd[wave] = h[C - 1];
-      __syncthreads();
-      if (wave > 0) left_in = xbnd[wave - 1];
-    }
-    fpr0 = wave_shr1(h[C - 1], left_in);
+      // h becomes G = F[r][c].
 #pragma unroll
-    for (int e = 0; e < C; ++e) {
-      Fp[e] = h[e];
-      tdc[e] = tdn[e];
+      for (int g = 0; g < G; ++g) {
+        const int32_t cg = min(carry, pre[g]);
+#pragma unroll
+        for (int e = 0; e < E; ++e) h[g * E + e] = TCOL(g, e) + min(cg, h[g * E + e]);
+      }
+
+      // Tree cells -> TD / TD^T; leaf+1 rows -> LDS slots.
+      uint16_t* wrow_ptr = slots + wslot * width;
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int j = g * E + e;
+          const int32_t t = TCOL(g, e);
+          if (t >= ncols) continue;
+          const int32_t c = cmax - 1 - t;
+          if (tree_row && fcol[j] == cmax) {
+            TDo[(int64_t)r * ldo + c] = h[j];
+            TDt[(int64_t)c * ldt + r] = h[j];
+          }
+          const int32_t widx1 = fw[j] & 0xFFFF;
+          if (wrow && widx1) wrow_ptr[widx1 - 1] = (uint16_t)h[j];
+        }
+      }
+
+      int32_t left_in = rmax - r;  // F[r][cmax]
+      if constexpr (W > 1) {
+        if (lane == 63) xbnd[wave] = h[C - 1];
+        __syncthreads();
+        if (wave > 0) left_in = xbnd[wave - 1];
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int32_t fill =
+            g ? __builtin_amdgcn_readlane(h[(g - 1) * E + E - 1], 63) : left_in;
+        fpr0[g] = wave_shr1(h[g * E + E - 1], fill);
+      }
+#pragma unroll
+      for (int j = 0; j < C; ++j) Fp[j] = h[j];
     }
+    }
+    mlab = nlab;
+    mrf = nrf;
+    mslot = nslot;
+    if (q + 2 < nchunks) load_meta(q + 2, nlab, nrf, nslot);
   }
+#undef TCOL
 }
 
 // ---------------------------------------------------------------------------
@@ -317,11 +458,6 @@ hipError_t forest_init_attributes(int limit) {
 // Launches one (class, task list). Returns false for a class without a kernel.
 bool launch_forest(int cls, const DPArgs& a, int lds_bytes, hipStream_t s) {
   if (a.ntasks <= 0) return true;
-  if (cls == kClassLane) {
-    const unsigned blocks = (unsigned)((a.ntasks + 255) / 256);
-    hipLaunchKernelGGL(lane_table_kernel, dim3(blocks), dim3(256), 0, s, a);
-    return true;
-  }
   if (cls >= 1 && cls <= kNumRowClasses) {
     const int wi = (cls - 1) / 5, ci = (cls - 1) % 5;
     switch (wi) {

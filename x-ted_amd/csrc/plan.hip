@@ -33,12 +33,14 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
                                          int* lds, uint32_t* i0, uint32_t* j0,
                                          unsigned long long* cells) {
   const int64_t kx = tx.node_off + k, ly = ty.node_off + l;
-  const int32_t a = A.pool.krsize[kx], b = A.pool.krsize[ly];
+  const int32_t a = A.pool.ssize[kx], b = A.pool.ssize[ly];
   *cells = (unsigned long long)a * (unsigned long long)b;
-  const int32_t kdx = A.pool.krkd[kx], kdy = A.pool.krkd[ly];
+  const int32_t kdx = A.pool.skd[kx], kdy = A.pool.skd[ly];
   const int32_t level = kdx + kdy;
-  int64_t need = (a <= b) ? (int64_t)kdx * (A.pool.krleaf[ly] - 1) * 2
-                          : (int64_t)kdy * (A.pool.krleaf[kx] - 1) * 2;
+  int64_t need = (a <= b) ? (int64_t)kdx * (A.pool.sleaf[ly] - 1) * 2
+                          : (int64_t)kdy * (A.pool.sleaf[kx] - 1) * 2;
+  // The row-scan kernels read slot 0 of every table unconditionally.
+  need = max(need, (int64_t)((a <= b ? A.pool.sleaf[ly] : A.pool.sleaf[kx]) - 1) * 2);
   int cls = table_class(a, b, need, A.rule);
   if (cls == kClassLane && !A.lane_enable) {
     // single-row tables routed to the narrowest row-scan class that fits
@@ -53,17 +55,19 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
   }
   *bucket = level * kNumClasses + cls;
   *lds = (int)need;
-  *i0 = (uint32_t)A.pool.kr[kx];
-  *j0 = (uint32_t)A.pool.kr[ly];
+  *i0 = (uint32_t)A.pool.skr[kx];
+  *j0 = (uint32_t)A.pool.skr[ly];
 }
 
 // Returns this lane's slot in cursors[bucket] after one wave-aggregated
 // atomicAdd per distinct bucket among the active lanes. With sums != null it
-// also adds the lanes' values of each bucket into sums[bucket].
+// also adds the lanes' values of each bucket into sums[bucket] and raises
+// maxes[bucket % kNumClasses] to the lanes' largest mval.
 __device__ __forceinline__ unsigned long long wave_reserve(unsigned long long* cursors,
                                                            unsigned long long* sums,
-                                                           int bucket, bool active,
-                                                           unsigned long long value) {
+                                                           int32_t* maxes, int bucket,
+                                                           bool active, unsigned long long value,
+                                                           int32_t mval) {
   unsigned long long pos = 0;
   const unsigned long long lt = __lanemask_lt();
   unsigned long long todo = __ballot(active);
@@ -79,8 +83,15 @@ __device__ __forceinline__ unsigned long long wave_reserve(unsigned long long* c
     if (mine) pos = base + __popcll(same & lt);
     if (sums) {
       unsigned long long v = mine ? value : 0ull;
-      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-      if (lane == leader) atomicAdd(&sums[lb], v);
+      int32_t mx = mine ? mval : 0;
+      for (int off = 32; off >= 1; off >>= 1) {
+        v += __shfl_xor(v, off);
+        mx = max(mx, __shfl_xor(mx, off));
+      }
+      if (lane == leader) {
+        atomicAdd(&sums[lb], v);
+        if (mx > 0) atomicMax(&maxes[lb % kNumClasses], mx);
+      }
     }
     todo &= ~same;
   }
@@ -93,7 +104,11 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(PlanArgs A) {
   const int p = find_pair(A.blk_start, A.npairs, blk);
   const PairMeta pm = A.pairs[p];
   const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
-  const int Kx = tx.K, Ky = ty.K;
+  // With the lane kernel on, tables with a leaf keyroot are not planned here
+  // (the lane kernel enumerates them per level): only the non-leaf keyroots
+  // [K1, K) of the sorted lists take part.
+  const int kbase = A.lane_enable ? tx.K1 : 0, lbase = A.lane_enable ? ty.K1 : 0;
+  const int Kx = tx.K - kbase, Ky = ty.K - lbase;
   const int lper = max(1, A.lchunk_tables / max(Kx, 1));  // same as plan_blocks()
   const int l_lo = (int)(blk - A.blk_start[p]) * lper;
   const int l_hi = min(Ky, l_lo + lper);
@@ -105,15 +120,14 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(PlanArgs A) {
     uint32_t i0 = 0, j0 = 0;
     unsigned long long cells = 0;
     if (active) {
-      const int l = l_lo + (int)(idx / Kx), k = (int)(idx % Kx);
+      const int l = lbase + l_lo + (int)(idx / Kx), k = kbase + (int)(idx % Kx);
       classify(A, tx, ty, k, l, &bucket, &lds, &i0, &j0, &cells);
     }
     if (kScatter) {
-      const unsigned long long pos = wave_reserve(A.cursor, nullptr, bucket, active, 0);
+      const unsigned long long pos = wave_reserve(A.cursor, nullptr, nullptr, bucket, active, 0, 0);
       if (active) A.tasks[pos] = task_pack((uint32_t)p, i0, j0);
     } else {
-      (void)wave_reserve(A.counts, A.cells, bucket, active, cells);
-      if (active && (bucket % kNumClasses) != kClassLane) atomicMax(&A.maxlds[bucket % kNumClasses], lds);
+      (void)wave_reserve(A.counts, A.cells, A.maxlds, bucket, active, cells, lds);
     }
   }
 }
@@ -158,9 +172,58 @@ void launch_plan(const PlanArgs& a, int64_t nblocks, hipStream_t s) {
   const int nb = a.nlevels * kNumClasses;
   hipLaunchKernelGGL(plan_clear_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, a.counts,
                      a.cells, nb, a.maxlds);
-  hipLaunchKernelGGL(plan_kernel<false>, dim3((unsigned)nblocks), dim3(kPlanThreads), 0, s, a);
+  if (nblocks > 0)
+    hipLaunchKernelGGL(plan_kernel<false>, dim3((unsigned)nblocks), dim3(kPlanThreads), 0, s, a);
   hipLaunchKernelGGL(plan_scan_kernel, dim3(1), dim3(1024), 0, s, a.counts, a.cursor, nb);
-  hipLaunchKernelGGL(plan_kernel<true>, dim3((unsigned)nblocks), dim3(kPlanThreads), 0, s, a);
+  if (nblocks > 0)
+    hipLaunchKernelGGL(plan_kernel<true>, dim3((unsigned)nblocks), dim3(kPlanThreads), 0, s, a);
+}
+
+// Tables with a leaf keyroot, per level L and pair p, form two rectangles
+// of the sorted keyroot lists: A = (leaf keyroots of x) x (keyroots of y at
+// height L) and B = (non-leaf keyroots of x at height L) x (leaf keyroots of
+// y). Their level is the other keyroot's height. One block per level writes
+// the exclusive prefix over pairs of |A| + |B| (npairs+1 entries) and the
+// level total.
+__global__ __launch_bounds__(1024) void lane_offsets_kernel(LaneArgs A) {
+  __shared__ unsigned long long part[1024];
+  const int L = blockIdx.x, tid = threadIdx.x;
+  const int np = A.npairs;
+  const int per = (np + 1023) / 1024;
+  const int lo = min(np, tid * per), hi = min(np, lo + per);
+  unsigned long long* out = A.offsets + (int64_t)L * (np + 1);
+  unsigned long long s = 0, cells = 0;
+  for (int p = lo; p < hi; ++p) {
+    int32_t sz[5];
+    s += lane_rect_count(A, p, L, sz);
+    cells += lane_rect_cells(A, p, sz);
+  }
+  part[tid] = s;
+  __shared__ unsigned long long csum;
+  if (tid == 0) csum = 0;
+  __syncthreads();
+  atomicAdd(&csum, cells);
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    unsigned long long v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  unsigned long long run = tid ? part[tid - 1] : 0;
+  for (int p = lo; p < hi; ++p) {
+    out[p] = run;
+    run += lane_rect_count(A, p, L, nullptr);
+  }
+  if (tid == 1023) {
+    out[np] = part[1023];
+    A.totals[L] = part[1023];
+    A.cells[L] = csum;
+  }
+}
+
+void launch_lane_offsets(const LaneArgs& a, int nlevels, hipStream_t s) {
+  hipLaunchKernelGGL(lane_offsets_kernel, dim3(nlevels), dim3(1024), 0, s, a);
 }
 
 // Distances TD[0][0] of every pair into dist[].

@@ -199,10 +199,49 @@ __global__ __launch_bounds__(kPrepThreads) void tree_prep_kernel(PrepArgs A) {
     int32_t i = A.kr[off + k];
     A.krleaf[off + k] = lrank[orl[i] + 1] - lrank[i];
   }
+  __syncthreads();
+
+  // ---- keyroots sorted for the scheduler -----------------------------------
+  // Leaf keyroots first, then the other keyroots by ascending nesting height
+  // (a stable counting sort over the descending list). One thread: this runs
+  // once per tree at upload.
   if (tid == 0) {
+    const int32_t kdmax = kdn[0];
+    int32_t* hist = anc0;  // free scratch, kdmax+2 <= n+1 entries
+    for (int d = 0; d <= kdmax + 1; ++d) hist[d] = 0;
+    int32_t K1 = 0;
+    for (int k = 0; k < K; ++k) {
+      const int32_t i = A.kr[off + k];
+      if (orl[i] == i) ++K1;
+      else ++hist[kdn[i]];
+    }
+    int32_t run = 0;
+    for (int d = 0; d <= kdmax; ++d) {
+      const int32_t c = hist[d];
+      A.kdoff[off + d] = run;
+      hist[d] = run;
+      run += c;
+    }
+    A.kdoff[off + kdmax + 1] = run;
+    int32_t leafpos = 0;
+    for (int k = 0; k < K; ++k) {
+      const int32_t i = A.kr[off + k];
+      const int32_t pos = (orl[i] == i) ? leafpos++ : K1 + hist[kdn[i]]++;
+      A.skr[off + pos] = i;
+      A.skd[off + pos] = kdn[i];
+      A.ssize[off + pos] = orl[i] - i + 1;
+      A.sleaf[off + pos] = lrank[orl[i] + 1] - lrank[i];
+    }
+    int32_t acc = 0;
+    for (int k = 0; k < K; ++k) {
+      A.spre[off + k] = acc;
+      acc += A.ssize[off + k];
+    }
+    A.spre[off + K] = acc;
     A.trees[t].K = K;
-    A.trees[t].kdmax = kdn[0];
+    A.trees[t].kdmax = kdmax;
     A.trees[t].nleaves = lrank[n];
+    A.trees[t].K1 = K1;
   }
 }
 

@@ -58,7 +58,7 @@ _SIGNATURES = {
     "xted_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32P, _I32P, _I32P,
                                   ctypes.c_void_p, ctypes.c_void_p]),
     "xted_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
-                                       ctypes.c_int32]),
+                                       ctypes.c_int32, ctypes.c_int32]),
     "xted_force_class": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "xted_set_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_class_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
@@ -255,8 +255,8 @@ class Context:
                                 ctypes.byref(st)))
         return dist, tds, st
 
-    def set_tuning(self, lds_per_wave, min_waves=1, wide_cols=1 << 30):
-        _check(lib().xted_set_tuning(self._h, lds_per_wave, min_waves, wide_cols))
+    def set_tuning(self, lds_per_wave=10 * 1024, max_cpl=8, min_waves=1, wide_cols=1 << 30):
+        _check(lib().xted_set_tuning(self._h, lds_per_wave, max_cpl, min_waves, wide_cols))
 
     def set_profile(self, on=True):
         _check(lib().xted_set_profile(self._h, 1 if on else 0))
