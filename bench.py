@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--plain-pairs", type=int, default=256)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="sweep the granularity rule first and use the fastest")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -96,6 +98,13 @@ def main():
     ctx = xted.Context(0)
     ids = ctx.pool_add(trees)
     xs, ys = ids[0::2].copy(), ids[1::2].copy()
+    import autotune
+    if args.autotune:
+        rule, _ = autotune.tune(ctx, xs, ys, log=log)
+    else:
+        rule = autotune.load()
+        autotune.apply(ctx, rule)
+    log(f"granularity rule: {rule}")
 
     dist = None
     for _ in range(args.warmup):
@@ -161,7 +170,7 @@ def main():
         "data": "synthetic (seeded generators, trees resident in HBM)",
         "config": {"workload": "batch4096_uneven_n100-1000_deep50_shallow50_alphabet200",
                    "pairs": args.pairs, "tables": st.tables, "cells": st.cells,
-                   "levels": st.levels, "launches": st.launches},
+                   "levels": st.levels, "launches": st.launches, "granularity_rule": rule},
         "gcups": round(gcups, 2),
         "plan_ms": round(st.plan_ms, 3),
         "dp_ms": round(st.dp_ms, 3),

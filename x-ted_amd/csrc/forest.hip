@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -39,14 +40,16 @@ constexpr int kBig = INT_MAX / 4;
 #define DPP_ROW_BCAST15 0x142
 #define DPP_ROW_BCAST31 0x143
 
-// Inclusive prefix minimum over the 64 lanes of a wave (lane order).
+// Inclusive prefix minimum over the 64 lanes of a wave (lane order). A lane
+// whose DPP source is out of range keeps its own value as the "old"
+// operand, which is the identity of min(v, v).
 __device__ __forceinline__ int wave_incl_min(int v) {
-  v = min(v, __builtin_amdgcn_update_dpp(kBig, v, DPP_ROW_SHR(1), 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(kBig, v, DPP_ROW_SHR(2), 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(kBig, v, DPP_ROW_SHR(4), 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(kBig, v, DPP_ROW_SHR(8), 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(kBig, v, DPP_ROW_BCAST15, 0xa, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(kBig, v, DPP_ROW_BCAST31, 0xc, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, DPP_ROW_SHR(1), 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, DPP_ROW_SHR(2), 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, DPP_ROW_SHR(4), 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, DPP_ROW_SHR(8), 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, DPP_ROW_BCAST15, 0xa, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, DPP_ROW_BCAST31, 0xc, 0xf, false));
   return v;
 }
 
@@ -83,44 +86,67 @@ __device__ __forceinline__ void lane_table(const PoolView& P, const PairMeta& pm
   const int32_t cmax = orlC[c0] + 1;
   const int32_t orl_c0 = cmax - 1;
   const int32_t labr = labR[r0];
-  // Loads run kLaneAhead columns ahead of the recurrence: slot k of the
-  // rings holds column c-k's orl, label and TD value. The loads are
-  // unconditional (past c0 they re-read column c0) and their values are
-  // only combined when used, so no load waits inside a branch.
-  constexpr int kLaneAhead = 4;
-  int32_t ring_o[kLaneAhead], ring_l[kLaneAhead], ring_v[kLaneAhead];
-  auto fetch = [&](int32_t c, int k) {
+  // Loads run kAhead columns ahead of the recurrence: ring slot k holds
+  // column c-k's orl, label and TD value. The main loop (c - kAhead >= c0)
+  // walks pointers and never clamps; the last kAhead columns are already in
+  // the ring. Values are only combined when used, so no load waits inside a
+  // branch, and the main loop is unrolled by the ring depth so the rotation
+  // is register renaming.
+  constexpr int kAhead = 4;
+  int32_t ring_o[kAhead], ring_l[kAhead], ring_v[kAhead];
+#pragma unroll
+  for (int k = 0; k < kAhead; ++k) {
+    const int32_t c = max(cmax - 1 - k, c0);
     ring_o[k] = orlC[c];
     ring_l[k] = labC[c];
     ring_v[k] = TDcr[(int64_t)c * ldcr + r0];
-  };
-#pragma unroll
-  for (int k = 0; k < kLaneAhead; ++k) fetch(max(cmax - 1 - k, c0), k);
+  }
+  const int32_t* po = orlC + (cmax - 1 - kAhead);
+  const int32_t* pl = labC + (cmax - 1 - kAhead);
+  const int32_t* pv = TDcr + (int64_t)(cmax - 1 - kAhead) * ldcr + r0;
+  int32_t* pw_cr = TDcr + (int64_t)(cmax - 1) * ldcr + r0;
+  int32_t* pw_rc = TDrc + (int64_t)r0 * ldrc + (cmax - 1);
   int32_t g = 1;  // F[r0][cmax] = imax - r0
-  // Unrolled by the ring depth so the ring rotation is register renaming,
-  // not copies of loads still in flight.
-#pragma unroll 4
-  for (int32_t c = cmax - 1; c >= c0; --c) {
-    const int32_t t = cmax - 1 - c;
+  int32_t t = 0;
+  auto step = [&]() {
     const int32_t oc = ring_o[0], lab = ring_l[0], v = ring_v[0];
 #pragma unroll
-    for (int k = 0; k + 1 < kLaneAhead; ++k) {
+    for (int k = 0; k + 1 < kAhead; ++k) {
       ring_o[k] = ring_o[k + 1];
       ring_l[k] = ring_l[k + 1];
       ring_v[k] = ring_v[k + 1];
     }
-    fetch(max(c - kLaneAhead, c0), kLaneAhead - 1);
     const bool tree = (oc == orl_c0);
     // tree cell: F[imax][c+1] + rename = t + (lab differ); forest cell:
-    // F[imax][orl(c)+1] + TD[r0][c] = (cmax - orl(c) - 1) + TD
-    const int32_t d = tree ? t + (labr != lab ? 1 : 0) : (cmax - oc - 1) + v;
-    const int32_t av = min(d, t + 2);
-    g = min(av, g + 1);
+    // F[imax][orl(c)+1] + TD[r0][c] = (cmax - orl(c) - 1) + TD. Both terms
+    // are formed and one is selected, so the step has no divergent branch.
+    const int32_t dt = t + (int32_t)(labr != lab);
+    const int32_t df = (cmax - oc - 1) + v;
+    const int32_t d = tree ? dt : df;
+    g = min(min(d, t + 2), g + 1);
     if (tree) {
-      TDrc[(int64_t)r0 * ldrc + c] = g;
-      TDcr[(int64_t)c * ldcr + r0] = g;
+      *pw_rc = g;
+      *pw_cr = g;
     }
+    pw_rc -= 1;
+    pw_cr -= ldcr;
+    ++t;
+  };
+  const int32_t ncols = cmax - c0;
+#pragma unroll 4
+  for (int32_t i = 0; i < ncols - kAhead; ++i) {
+    const int32_t o = *po, l = *pl, v = *pv;
+    step();
+    ring_o[kAhead - 1] = o;
+    ring_l[kAhead - 1] = l;
+    ring_v[kAhead - 1] = v;
+    po -= 1;
+    pl -= 1;
+    pv -= ldcr;
   }
+#pragma unroll
+  for (int k = 0; k < kAhead; ++k)
+    if (t < ncols) step();
 }
 
 // Single-row tables of one level, without a task list: thread g finds its
@@ -176,8 +202,8 @@ typedef int32_t int2u __attribute__((ext_vector_type(2), aligned(4)));
 
 template <int W, int C>
 __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
-  constexpr int E = C <= 4 ? C : 4;               // contiguous columns per group
-  constexpr int G = C / E;                        // groups per lane
+  constexpr int E = C <= 4 ? C : 4;  // contiguous columns per group
+  constexpr int G = C / E;           // groups per lane
   constexpr int S = C <= 8 ? 4 : 2;  // TD ring slots (divides 64)
   constexpr int PD = S - 1;          // rows of TD prefetched
   extern __shared__ uint16_t slots[];
@@ -212,20 +238,30 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   const int32_t base = kdepR[r0];
   const int32_t orl_c0 = cmax - 1;
   const int32_t lr_c0 = lrankC[c0];
+  // A slot row holds the compact far columns (leaves of subtree(c0) but the
+  // last) and, at index width, F[R][cmax] = rmax - R for the tree columns.
   const int32_t width = lrankC[cmax] - lr_c0 - 1;
+  const int32_t rowlen = width + 1;
 
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int32_t wbase = wave * 64 * C;
-#define TCOL(g, e) (wbase + (g) * 64 * E + lane * E + (e))
+  const int32_t tlane = wave * 64 * C + lane * E;  // t of this lane's element (0, 0)
+#define TCOL(g, e) (tlane + (g) * 64 * E + (e))
 
-  // Per-column state, element j = g*E + e. fcol = orl_C(c)+1, so the column
-  // is a tree column exactly when fcol == cmax. fw packs the compact far
-  // index (high 16 bits) and 1 + the saved-row write index (low 16 bits, 0 =
-  // not a leaf+1 column). Fp holds F[r+1][c]; F[r+1][c+1] is Fp of element
-  // j-1, or fpr0[g] for e = 0.
-  int32_t labc[C], fcol[C], fw[C], Fp[C], tdq[S][C];
-  int32_t fpr0[G];
+  // Per-column state, element j = g*E + e (t = TCOL(g, e), c = cmax-1-t).
+  //   bfar: F[rmax][orl_C(c)+1] = cmax - orl_C(c) - 1 (the far value on the
+  //         boundary row), or -1 on a tree column (orl_C(c) == orl_C(c0));
+  //   labc: the column label (tree cells only);
+  //   foff: the column's far index in a slot row (width for tree columns);
+  //   woff: the index this column writes in a saved row (c-1 a leaf of
+  //         subtree(c0)), or -1;
+  //   Fpm:  F[r+1][c] + 1 - t (the scan works on a[t] - t);
+  //   negt: -t.
+  // Columns past the table (t >= ncols) come after every real column in the
+  // scan order, so whatever they hold never reaches a real column; only
+  // their stores are masked.
+  int32_t bfar[C], labc[C], foff[C], woff[C], Fpm[C], negt[C], tdq[S][C];
+  int32_t fpr0[G];  // F[r+1][c+1] of each group's element e = 0
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     fpr0[g] = TCOL(g, 0);  // F[rmax][c+1] = cmax-1-c = t
@@ -233,19 +269,19 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     for (int e = 0; e < E; ++e) {
       const int j = g * E + e;
       const int32_t t = TCOL(g, e);
-      const int32_t c = cmax - 1 - t;
-      labc[j] = 0;
-      fcol[j] = cmax;
-      fw[j] = 0;
-      if (t < ncols) {
-        const int32_t oc = orlC[c];
-        labc[j] = labC[c];
-        fcol[j] = oc + 1;
-        const int32_t fidx = (oc == orl_c0) ? 0 : lrankC[oc] - lr_c0;
-        const int32_t widx1 = (c > c0 && orlC[c - 1] == c - 1) ? lrankC[c - 1] - lr_c0 + 1 : 0;
-        fw[j] = (fidx << 16) | widx1;
-      }
-      Fp[j] = t + 1;  // F[rmax][c] = cmax - c
+      const int32_t c = max(cmax - 1 - t, c0);
+      const int32_t oc = orlC[c];
+      const int32_t lab = labC[c];
+      const int32_t lro = lrankC[oc];
+      const int32_t ocm = orlC[max(c - 1, c0)];
+      const int32_t lrm = lrankC[max(c - 1, c0)];
+      const bool tree = (oc == orl_c0);
+      bfar[j] = tree ? -1 : cmax - oc - 1;
+      labc[j] = lab;
+      foff[j] = tree ? width : lro - lr_c0;
+      woff[j] = (t < ncols && c > c0 && ocm == c - 1) ? lrm - lr_c0 : -1;
+      Fpm[j] = 2;  // F[rmax][c] + 1 - t = cmax - c + 1 - t
+      negt[j] = -t;
     }
   }
 
@@ -253,8 +289,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // the compiler never waits for them inside a branch: a group with no valid
   // column reads the row's first E ints instead, a partly valid group reads
   // up to E-1 ints before column c0 (the arena keeps kTdPad ints of padding
-  // before and after every table, so both stay inside the allocation), and
-  // invalid elements are masked afterwards.
+  // before and after every table, so both stay inside the allocation).
   auto load_row = [&](int32_t rr, int32_t* out) {
     const int32_t* row = TDo + (int64_t)rr * ldo;
 #pragma unroll
@@ -284,12 +319,13 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // per row with readlane; the next chunk is loaded one chunk ahead.
   auto load_meta = [&](int q, int32_t& mlab, int32_t& mrf, int32_t& mslot) {
     const int32_t rr = rmax - 1 - (64 * q + lane);
-    const int32_t r = max(rr, r0);          // rows past the table read row r0
+    const int32_t r = max(rr, r0);  // rows past the table read row r0
     const int32_t rp = max(r - 1, r0);
     const int32_t orr = orlR[r], lab = labR[r], kd = kdepR[r];
     const int32_t orp = orlR[rp], kdp = kdepR[rp];
     const int32_t fs = (orr + 1 < rmax) ? kd - base - 1 : 0;
-    const int32_t ws = (r > r0 && orp == r - 1) ? (((kdp - base - 1) << 16) | (int32_t)0x80000000) : 0;
+    const int32_t ws =
+        (r > r0 && orp == r - 1) ? (((kdp - base - 1) << 16) | (int32_t)0x80000000) : 0;
     mlab = lab;
     mrf = orr + 1;
     mslot = fs | ws;
@@ -308,97 +344,108 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     // the previous row has just finished with.
     for (int32_t i0r = 0; i0r < rows_here; i0r += S) {
 #pragma unroll
-    for (int u = 0; u < S; ++u) {
-      const int32_t i = i0r + u;
-      if (i >= rows_here) break;
-      const int32_t r = rmax - 1 - (64 * q + i);
-      load_row(max(r - PD, r0), tdq[(u + PD) % S]);
+      for (int u = 0; u < S; ++u) {
+        const int32_t i = i0r + u;
+        if (i >= rows_here) break;
+        const int32_t r = rmax - 1 - (64 * q + i);
+        load_row(max(r - PD, r0), tdq[(u + PD) % S]);
 
-      const int32_t labr = __builtin_amdgcn_readlane(mlab, i);
-      const int32_t Rf = __builtin_amdgcn_readlane(mrf, i);
-      const int32_t sl = __builtin_amdgcn_readlane(mslot, i);
-      const bool tree_row = (Rf == rmax);
-      const int32_t fslot = sl & 0xFFFF;
-      const bool wrow = sl < 0;
-      const int32_t wslot = (sl >> 16) & 0x7FFF;
-      const uint16_t* frow = slots + fslot * width;
+        const int32_t labr = __builtin_amdgcn_readlane(mlab, i);
+        const int32_t Rf = __builtin_amdgcn_readlane(mrf, i);
+        const int32_t sl = __builtin_amdgcn_readlane(mslot, i);
+        const bool tree_row = (Rf == rmax);
+        const uint16_t* frow = slots + (sl & 0xFFFF) * rowlen;
+        const bool wrow = sl < 0;
+        uint16_t* wrow_ptr = slots + ((sl >> 16) & 0x7FFF) * rowlen;
 
-      // Far-row values for every element, read unconditionally (index 0 of
-      // slot 0 when unused) so the C LDS reads issue back to back.
-      int32_t sv[C];
+        // h = a[t] - t per element, local prefix per group, chained wave
+        // scans. Tree rows (orl_R(r) == orl_R(r0)) and the other rows take
+        // separate code paths on the uniform flag.
+        int32_t h[C], pre[G];
+        int32_t carry_w = kBig;  // min over this wave's earlier groups
+        auto row_terms = [&](auto tree_tag) {
+          constexpr bool kTree = decltype(tree_tag)::value;
+          int32_t sv[C];
+          if constexpr (!kTree) {
 #pragma unroll
-      for (int j = 0; j < C; ++j) sv[j] = (int32_t)frow[(uint32_t)fw[j] >> 16];
-
-      // a[t] - t per element, local prefix per group, chained wave scans.
-      int32_t h[C], pre[G];
-      int32_t carry_w = kBig;  // min over this wave's earlier groups
-#pragma unroll
-      for (int g = 0; g < G; ++g) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const int j = g * E + e;
-          const int32_t t = TCOL(g, e);
-          const bool tree_col = (fcol[j] == cmax);
-          const int32_t fpr = e ? Fp[j - 1] : fpr0[g];
-          const int32_t dtree = fpr + (labr != labc[j] ? 1 : 0);
-          const int32_t farv = (Rf == rmax) ? cmax - fcol[j] : (tree_col ? rmax - Rf : sv[j]);
-          const int32_t d = (tree_row && tree_col) ? dtree : farv + tdq[u][j];
-          const int32_t av = min(d, Fp[j] + 1);
-          h[j] = (t < ncols) ? av - t : kBig;
-          if (e) h[j] = min(h[j], h[j - 1]);
-        }
-        const int32_t incl = wave_incl_min(h[g * E + E - 1]);
-        pre[g] = min(carry_w, wave_shr1(incl, kBig));
-        carry_w = min(carry_w, __builtin_amdgcn_readlane(incl, 63));
-      }
-      int32_t carry = (rmax - r) + 1;  // F[r][cmax] - (-1)
-      if constexpr (W > 1) {
-        if (lane == 0) xtot[wave] = carry_w;
-        __syncthreads();
-        for (int w = 0; w < wave; ++w) carry = min(carry, xtot[w]);
-      }
-      // h becomes G = F[r][c].
-#pragma unroll
-      for (int g = 0; g < G; ++g) {
-        const int32_t cg = min(carry, pre[g]);
-#pragma unroll
-        for (int e = 0; e < E; ++e) h[g * E + e] = TCOL(g, e) + min(cg, h[g * E + e]);
-      }
-
-      // Tree cells -> TD / TD^T; leaf+1 rows -> LDS slots.
-      uint16_t* wrow_ptr = slots + wslot * width;
-#pragma unroll
-      for (int g = 0; g < G; ++g) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const int j = g * E + e;
-          const int32_t t = TCOL(g, e);
-          if (t >= ncols) continue;
-          const int32_t c = cmax - 1 - t;
-          if (tree_row && fcol[j] == cmax) {
-            TDo[(int64_t)r * ldo + c] = h[j];
-            TDt[(int64_t)c * ldt + r] = h[j];
+            for (int j = 0; j < C; ++j) sv[j] = frow[foff[j]];
           }
-          const int32_t widx1 = fw[j] & 0xFFFF;
-          if (wrow && widx1) wrow_ptr[widx1 - 1] = (uint16_t)h[j];
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+              const int j = g * E + e;
+              int32_t dm;  // diagonal-or-far term minus t
+              if constexpr (kTree) {
+                // diagonal F[r+1][c+1] + rename on a tree column, else the
+                // boundary-row far value. F[r+1][c+1] - t = Fpm[j-1] - 2.
+                const int32_t fprm = e ? Fpm[j - 1] - 2 : fpr0[g] + negt[j];
+                dm = bfar[j] < 0 ? fprm + (int32_t)(labr != labc[j]) : bfar[j] + tdq[u][j] + negt[j];
+              } else {
+                dm = sv[j] + tdq[u][j] + negt[j];
+              }
+              h[j] = min(dm, Fpm[j]);
+              if (e) h[j] = min(h[j], h[j - 1]);
+            }
+            const int32_t incl = wave_incl_min(h[g * E + E - 1]);
+            pre[g] = min(carry_w, wave_shr1(incl, kBig));
+            carry_w = min(carry_w, __builtin_amdgcn_readlane(incl, 63));
+          }
+        };
+        if (tree_row) row_terms(std::true_type{});
+        else row_terms(std::false_type{});
+
+        int32_t carry = (rmax - r) + 1;  // F[r][cmax] - (-1)
+        if constexpr (W > 1) {
+          if (lane == 0) xtot[wave] = carry_w;
+          __syncthreads();
+          for (int w = 0; w < wave; ++w) carry = min(carry, xtot[w]);
+        }
+        // h becomes G - t; Fpm = G + 1 - t = h + 1.
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const int32_t cg = min(carry, pre[g]);
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+            const int j = g * E + e;
+            h[j] = min(cg, h[j]);
+            Fpm[j] = h[j] + 1;
+          }
+        }
+
+        // Tree cells -> TD / TD^T (tree rows only); saved rows -> LDS slots.
+        if (tree_row) {
+#pragma unroll
+          for (int j = 0; j < C; ++j) {
+            const int32_t t = TCOL(j / E, j % E);
+            if (t < ncols && bfar[j] < 0) {
+              const int32_t c = cmax - 1 - t;
+              TDo[(int64_t)r * ldo + c] = h[j] + t;
+              TDt[(int64_t)c * ldt + r] = h[j] + t;
+            }
+          }
+        }
+        if (wrow) {
+#pragma unroll
+          for (int j = 0; j < C; ++j)
+            if (woff[j] >= 0) wrow_ptr[woff[j]] = (uint16_t)(h[j] - negt[j]);
+          if (threadIdx.x == 0) wrow_ptr[width] = (uint16_t)(rmax - r);
+        }
+
+        int32_t left_in = rmax - r;  // F[r][cmax]
+        if constexpr (W > 1) {
+          if (lane == 63) xbnd[wave] = h[C - 1] - negt[C - 1];
+          __syncthreads();
+          if (wave > 0) left_in = xbnd[wave - 1];
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const int jl = g * E + E - 1;  // the group's last column
+          const int32_t fill =
+              g ? __builtin_amdgcn_readlane(h[jl - E] - negt[jl - E], 63) : left_in;
+          fpr0[g] = wave_shr1(h[jl] - negt[jl], fill);
         }
       }
-
-      int32_t left_in = rmax - r;  // F[r][cmax]
-      if constexpr (W > 1) {
-        if (lane == 63) xbnd[wave] = h[C - 1];
-        __syncthreads();
-        if (wave > 0) left_in = xbnd[wave - 1];
-      }
-#pragma unroll
-      for (int g = 0; g < G; ++g) {
-        const int32_t fill =
-            g ? __builtin_amdgcn_readlane(h[(g - 1) * E + E - 1], 63) : left_in;
-        fpr0[g] = wave_shr1(h[g * E + E - 1], fill);
-      }
-#pragma unroll
-      for (int j = 0; j < C; ++j) Fp[j] = h[j];
-    }
     }
     mlab = nlab;
     mrf = nrf;

@@ -37,10 +37,11 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
   *cells = (unsigned long long)a * (unsigned long long)b;
   const int32_t kdx = A.pool.skd[kx], kdy = A.pool.skd[ly];
   const int32_t level = kdx + kdy;
-  int64_t need = (a <= b) ? (int64_t)kdx * (A.pool.sleaf[ly] - 1) * 2
-                          : (int64_t)kdy * (A.pool.sleaf[kx] - 1) * 2;
-  // The row-scan kernels read slot 0 of every table unconditionally.
-  need = max(need, (int64_t)((a <= b ? A.pool.sleaf[ly] : A.pool.sleaf[kx]) - 1) * 2);
+  // Slot storage of the oriented table: one uint16 row per nesting level of
+  // the row keyroot (at least one: the kernels read slot 0 unconditionally),
+  // each as long as the column keyroot's leaf count.
+  const int32_t slots = max(a <= b ? kdx : kdy, 1);
+  const int64_t need = (int64_t)slots * (a <= b ? A.pool.sleaf[ly] : A.pool.sleaf[kx]) * 2;
   int cls = table_class(a, b, need, A.rule);
   if (cls == kClassLane && !A.lane_enable) {
     // single-row tables routed to the narrowest row-scan class that fits
@@ -59,15 +60,35 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
   *j0 = (uint32_t)A.pool.skr[ly];
 }
 
-// Returns this lane's slot in cursors[bucket] after one wave-aggregated
-// atomicAdd per distinct bucket among the active lanes. With sums != null it
-// also adds the lanes' values of each bucket into sums[bucket] and raises
-// maxes[bucket % kNumClasses] to the lanes' largest mval.
-__device__ __forceinline__ unsigned long long wave_reserve(unsigned long long* cursors,
-                                                           unsigned long long* sums,
-                                                           int32_t* maxes, int bucket,
-                                                           bool active, unsigned long long value,
-                                                           int32_t mval) {
+#define DPP_ROW_SHR(n) (0x110 + (n))
+#define DPP_ROW_BCAST15 0x142
+#define DPP_ROW_BCAST31 0x143
+
+// Wave-wide sum / max of 32-bit values, result in every lane (DPP prefix,
+// then lane 63 broadcast).
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+  v += __builtin_amdgcn_update_dpp(0u, v, DPP_ROW_SHR(1), 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, DPP_ROW_SHR(2), 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, DPP_ROW_SHR(4), 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, DPP_ROW_SHR(8), 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, DPP_ROW_BCAST15, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0u, v, DPP_ROW_BCAST31, 0xc, 0xf, false);
+  return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(1), 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(2), 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(4), 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(8), 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST15, 0xa, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST31, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Scatter pass: this lane's slot in cursors[bucket], after one
+// wave-aggregated atomicAdd per distinct bucket among the active lanes.
+__device__ __forceinline__ unsigned long long wave_reserve(unsigned long long* cursors, int bucket,
+                                                           bool active) {
   unsigned long long pos = 0;
   const unsigned long long lt = __lanemask_lt();
   unsigned long long todo = __ballot(active);
@@ -81,21 +102,34 @@ __device__ __forceinline__ unsigned long long wave_reserve(unsigned long long* c
     if (lane == leader) base = atomicAdd(&cursors[lb], (unsigned long long)__popcll(same));
     base = __shfl(base, leader);
     if (mine) pos = base + __popcll(same & lt);
-    if (sums) {
-      unsigned long long v = mine ? value : 0ull;
-      int32_t mx = mine ? mval : 0;
-      for (int off = 32; off >= 1; off >>= 1) {
-        v += __shfl_xor(v, off);
-        mx = max(mx, __shfl_xor(mx, off));
-      }
-      if (lane == leader) {
-        atomicAdd(&sums[lb], v);
-        if (mx > 0) atomicMax(&maxes[lb % kNumClasses], mx);
-      }
-    }
     todo &= ~same;
   }
   return pos;
+}
+
+// Count pass: per distinct bucket of the wave, one fire-and-forget atomic
+// each for the table count, the interior cells (a table has < 2^30 cells;
+// the wave sum is taken over 16-bit halves so 32-bit sums cannot overflow)
+// and the largest slot-storage need of the bucket's class.
+__device__ __forceinline__ void wave_count(const PlanArgs& A, int bucket, bool active,
+                                           uint32_t cells, int32_t lds) {
+  unsigned long long todo = __ballot(active);
+  const int lane = (int)(threadIdx.x & 63);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lb = __shfl(bucket, leader);
+    const bool mine = active && bucket == lb;
+    const unsigned long long same = __ballot(mine);
+    const uint32_t lo = wave_sum_u32(mine ? (cells & 0xFFFFu) : 0u);
+    const uint32_t hi = wave_sum_u32(mine ? (cells >> 16) : 0u);
+    const int32_t mx = wave_max_i32(mine ? lds : 0);
+    if (lane == leader) {
+      atomicAdd(&A.counts[lb], (unsigned long long)__popcll(same));
+      atomicAdd(&A.cells[lb], ((unsigned long long)hi << 16) + lo);
+      if (mx > 0) atomicMax(&A.maxlds[lb % kNumClasses], mx);
+    }
+    todo &= ~same;
+  }
 }
 
 template <bool kScatter>
@@ -124,10 +158,10 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(PlanArgs A) {
       classify(A, tx, ty, k, l, &bucket, &lds, &i0, &j0, &cells);
     }
     if (kScatter) {
-      const unsigned long long pos = wave_reserve(A.cursor, nullptr, nullptr, bucket, active, 0, 0);
+      const unsigned long long pos = wave_reserve(A.cursor, bucket, active);
       if (active) A.tasks[pos] = task_pack((uint32_t)p, i0, j0);
     } else {
-      (void)wave_reserve(A.counts, A.cells, A.maxlds, bucket, active, cells, lds);
+      wave_count(A, bucket, active, (uint32_t)cells, lds);
     }
   }
 }
