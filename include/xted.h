@@ -108,6 +108,11 @@ int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t max_cpl,
 /* Forces every table that fits into class cls (XTED_CLASS_AUTO restores the
  * rule); lane_enable = 0 sends single-row tables to the row-scan kernels. */
 int xted_force_class(xted_context* ctx, int32_t cls, int32_t lane_enable);
+/* Streams (1 to 4) that the launches of one level run on concurrently. The
+ * reference gives each class kernel its own stream but waits for each
+ * launch before the next (X-TED_GPU/parallel_X_TED.cu:433-438). Results do
+ * not depend on it. */
+int xted_set_level_streams(xted_context* ctx, int32_t n);
 
 /* ---- per-class device timing ---------------------------------------------------
  * With profiling on, xted_ted / xted_batch bracket every forest-DP launch

@@ -167,3 +167,50 @@ def test_batch_matches_single_and_oracle(gpu_ctx):
         od, otd, _ = oracle.ted(trees[a], trees[b], threads=4)
         assert dist[k] == od
         assert (tds[k] == otd).all()
+
+
+def _mixed_batch(ctx, npairs, seed):
+    rng = np.random.default_rng(seed)
+    trees = [gen(int(rng.integers(0, 3)), int(rng.integers(2, 700)), 30, 80 + seed, i)
+             for i in range(2 * npairs)]
+    ctx.pool_clear()
+    ids = ctx.pool_add(trees)
+    shapes = [(len(trees[2 * k][0]), len(trees[2 * k + 1][0])) for k in range(npairs)]
+    return trees, ids[0::2].copy(), ids[1::2].copy(), shapes
+
+
+def test_level_streams_do_not_change_results(gpu_ctx):
+    """Launches of one level spread over 1, 2 or 4 streams: tables identical,
+    distances equal to the oracle's."""
+    trees, xs, ys, shapes = _mixed_batch(gpu_ctx, 40, 1)
+    out = {}
+    try:
+        for n in (1, 2, 4):
+            gpu_ctx.set_level_streams(n)
+            out[n] = gpu_ctx.batch(xs, ys, td_shapes=shapes)
+    finally:
+        gpu_ctx.set_level_streams(4)
+    for n in (2, 4):
+        assert np.array_equal(out[n][0], out[1][0])
+        for a, b in zip(out[n][1], out[1][1]):
+            assert np.array_equal(a, b)
+    for k in range(0, 40, 5):
+        assert out[4][0][k] == oracle.ted(trees[2 * k], trees[2 * k + 1], threads=4)[0]
+
+
+@pytest.mark.parametrize("rule", [(10240, 4), (20480, 16), (163840, 8)])
+def test_granularity_rules_do_not_change_results(gpu_ctx, rule):
+    """Autotuner candidates re-class tables; every table must stay bit-exact."""
+    trees, xs, ys, shapes = _mixed_batch(gpu_ctx, 24, 2)
+    base = gpu_ctx.batch(xs, ys, td_shapes=shapes)
+    try:
+        gpu_ctx.set_tuning(lds_per_wave=rule[0], max_cpl=rule[1])
+        got = gpu_ctx.batch(xs, ys, td_shapes=shapes)
+    finally:
+        gpu_ctx.set_tuning()
+    assert np.array_equal(got[0], base[0])
+    for a, b in zip(got[1], base[1]):
+        assert np.array_equal(a, b)
+    for k in range(0, 24, 6):
+        od, otd, _ = oracle.ted(trees[2 * k], trees[2 * k + 1], threads=4)
+        assert got[0][k] == od and np.array_equal(got[1][k], otd)

@@ -84,6 +84,11 @@ Context::~Context() {
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : prof_ev_) (void)hipEventDestroy(e);
+  if (fork_ev_) (void)hipEventDestroy(fork_ev_);
+  for (auto& e : join_ev_)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& sd : side_)
+    if (sd) (void)hipStreamDestroy(sd);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -101,7 +106,10 @@ int Context::init() {
   }
   HIP_TRY(hipSetDevice(device_));
   HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  for (auto& sd : side_) HIP_TRY(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
   for (auto& ev : ev_) HIP_TRY(hipEventCreate(&ev));
+  HIP_TRY(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+  for (auto& ev : join_ev_) HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   // Slot storage limit: what the device grants one workgroup, less the
   // row-scan kernels' static LDS, capped at kLdsMaxTable.
   hipDeviceProp_t prop;
@@ -316,7 +324,7 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   ST_TRY(counts_.grow(sizeof(unsigned long long) * nb, false, stream_));
   ST_TRY(cells_.grow(sizeof(unsigned long long) * nb, false, stream_));
   ST_TRY(cursor_.grow(sizeof(unsigned long long) * nb, false, stream_));
-  ST_TRY(maxlds_.grow(sizeof(int32_t) * kNumClasses, false, stream_));
+  ST_TRY(maxlds_.grow(sizeof(int32_t) * nb, false, stream_));
   ST_TRY(tasks_.grow(sizeof(uint64_t) * std::max<int64_t>(planned, 1), false, stream_));
   ST_TRY(td_.grow(sizeof(int32_t) * td_total, false, stream_));
   ST_TRY(dist_.grow(sizeof(int32_t) * npairs, false, stream_));
@@ -366,14 +374,14 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
                            sizeof(unsigned long long) * nlevels, hipMemcpyDeviceToHost, stream_));
   }
   h_counts_.assign(nb, 0);
-  int32_t h_maxlds[kNumClasses] = {0};
+  h_maxlds_.assign(nb, 0);
   HIP_TRY(hipMemcpyAsync(h_counts_.data(), counts_.p, sizeof(unsigned long long) * nb,
                          hipMemcpyDeviceToHost, stream_));
-  HIP_TRY(hipMemcpyAsync(h_maxlds, maxlds_.p, sizeof(h_maxlds), hipMemcpyDeviceToHost, stream_));
+  HIP_TRY(hipMemcpyAsync(h_maxlds_.data(), maxlds_.p, sizeof(int32_t) * nb,
+                         hipMemcpyDeviceToHost, stream_));
   h_cells_.assign(nb, 0);
-  if (profile)
-    HIP_TRY(hipMemcpyAsync(h_cells_.data(), cells_.p, sizeof(unsigned long long) * nb,
-                           hipMemcpyDeviceToHost, stream_));
+  HIP_TRY(hipMemcpyAsync(h_cells_.data(), cells_.p, sizeof(unsigned long long) * nb,
+                         hipMemcpyDeviceToHost, stream_));
   HIP_TRY(hipStreamSynchronize(stream_));
   HIP_TRY(hipEventRecord(ev_[1], stream_));
   for (int L = 0; L < nlevels; ++L) {
@@ -400,36 +408,54 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
       prof_ev_.push_back(e);
     }
   }
+  // Launches of one level are independent: they fork from the main stream
+  // onto up to level_streams streams (largest first) and join before the
+  // next level, so latency-bound launches of few large tables overlap with
+  // the rest of their level instead of queueing behind it.
+  struct Launch {
+    int cls, bucket;
+    unsigned long long off, cnt, cells;
+  };
+  std::vector<Launch> lv;
   unsigned long long off = 0;
   for (int L = 0; L < nlevels; ++L) {
-    if (h_lane_tot_[L]) {
-      la.level = L;
-      if (profile) HIP_TRY(hipEventRecord(prof_ev_[2 * prof.size()], stream_));
-      launch_lane_level(la, h_lane_tot_[L], stream_);
+    lv.clear();
+    if (h_lane_tot_[L]) lv.push_back({kClassLane, -1 - L, 0, h_lane_tot_[L], h_lane_cells_[L]});
+    for (int c = 0; c < kNumClasses; ++c) {
+      const size_t b = (size_t)L * kNumClasses + c;
+      if (h_counts_[b]) lv.push_back({c, (int)b, off, h_counts_[b], h_cells_[b]});
+      off += h_counts_[b];
+    }
+    std::stable_sort(lv.begin(), lv.end(),
+                     [](const Launch& x, const Launch& y) { return x.cells > y.cells; });
+    const int ns = std::min<int>((int)lv.size(), std::max(1, level_streams));
+    if (ns > 1) HIP_TRY(hipEventRecord(fork_ev_, stream_));
+    for (int k = 1; k < ns; ++k) HIP_TRY(hipStreamWaitEvent(side_[k - 1], fork_ev_, 0));
+    for (size_t i = 0; i < lv.size(); ++i) {
+      const Launch& w = lv[i];
+      hipStream_t s = (i % ns) == 0 ? stream_ : side_[i % ns - 1];
+      if (profile) HIP_TRY(hipEventRecord(prof_ev_[2 * prof.size()], s));
+      if (w.cls == kClassLane) {
+        la.level = L;
+        launch_lane_level(la, w.cnt, s);
+      } else {
+        da.tasks = tasks_.as<uint64_t>() + w.off;
+        da.ntasks = (int64_t)w.cnt;
+        const int lds = std::max(64, (h_maxlds_[w.bucket] + 15) & ~15);
+        if (!launch_forest(w.cls, da, lds, s)) {
+          set_error("run_batch: no kernel for class " + std::to_string(w.cls));
+          return ERR_INVALID_ARG;
+        }
+      }
       if (profile) {
-        HIP_TRY(hipEventRecord(prof_ev_[2 * prof.size() + 1], stream_));
-        prof.emplace_back(kClassLane, -1 - L);
+        HIP_TRY(hipEventRecord(prof_ev_[2 * prof.size() + 1], s));
+        prof.emplace_back(w.cls, w.bucket);
       }
       ++st.launches;
     }
-    for (int c = 0; c < kNumClasses; ++c) {
-      const unsigned long long cnt = h_counts_[(size_t)L * kNumClasses + c];
-      if (cnt) {
-        da.tasks = tasks_.as<uint64_t>() + off;
-        da.ntasks = (int64_t)cnt;
-        const int lds = std::max(64, (h_maxlds[c] + 15) & ~15);
-        if (profile) HIP_TRY(hipEventRecord(prof_ev_[2 * prof.size()], stream_));
-        if (!launch_forest(c, da, lds, stream_)) {
-          set_error("run_batch: no kernel for class " + std::to_string(c));
-          return ERR_INVALID_ARG;
-        }
-        if (profile) {
-          HIP_TRY(hipEventRecord(prof_ev_[2 * prof.size() + 1], stream_));
-          prof.emplace_back(c, L * kNumClasses + c);
-        }
-        ++st.launches;
-      }
-      off += cnt;
+    for (int k = 1; k < ns; ++k) {
+      HIP_TRY(hipEventRecord(join_ev_[k - 1], side_[k - 1]));
+      HIP_TRY(hipStreamWaitEvent(stream_, join_ev_[k - 1], 0));
     }
   }
   HIP_TRY(hipGetLastError());

@@ -17,6 +17,11 @@ namespace xted {
 void set_error(const std::string& msg);
 const char* last_error();
 
+// Streams (the context's own plus side streams) that the launches of one
+// level are spread over; four matches the hardware queues HIP gives a
+// process by default.
+constexpr int kMaxLevelStreams = 4;
+
 struct BatchStats {
   double plan_ms = 0;      // device plan build incl. the count read-back
   double dp_ms = 0;        // level loop (all forest kernels)
@@ -81,6 +86,7 @@ class Context {
   int32_t lane_enable = 1;
   int32_t plan_chunk = 2048;  // tables per plan block
   int32_t profile = 0;        // time every forest launch with events
+  int32_t level_streams = kMaxLevelStreams;  // streams one level's launches spread over
   ClassProfile class_profile[kNumClasses];
 
   hipStream_t stream() const { return stream_; }
@@ -89,7 +95,9 @@ class Context {
  private:
   int device_;
   hipStream_t stream_ = nullptr;
+  hipStream_t side_[kMaxLevelStreams - 1] = {};
   hipEvent_t ev_[4] = {};
+  hipEvent_t fork_ev_ = nullptr, join_ev_[kMaxLevelStreams - 1] = {};
   std::mutex mu_;
   // pool
   std::vector<TreeMeta> meta_;
@@ -102,6 +110,7 @@ class Context {
   DevBuf pairs_, blk_, counts_, cells_, cursor_, maxlds_, tasks_, td_, dist_;
   DevBuf lane_off_, lane_tot_, lane_cells_;
   std::vector<unsigned long long> h_counts_, h_cells_, h_lane_tot_, h_lane_cells_;
+  std::vector<int32_t> h_maxlds_;  // largest slot storage per (level, class) bucket
   std::vector<hipEvent_t> prof_ev_;
   int grow_pool(int64_t nodes);
   PoolView view() const;

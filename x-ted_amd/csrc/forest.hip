@@ -204,11 +204,14 @@ template <int W, int C>
 __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   constexpr int E = C <= 4 ? C : 4;  // contiguous columns per group
   constexpr int G = C / E;           // groups per lane
-  constexpr int S = C <= 8 ? 4 : 2;  // TD ring slots (divides 64)
-  constexpr int PD = S - 1;          // rows of TD prefetched
+  // TD ring slots (a power of two dividing 64): up to 32 registers of ring,
+  // so narrow lanes prefetch more rows ahead.
+  constexpr int S = C >= 16 ? 2 : C >= 8 ? 4 : C >= 4 ? 8 : 16;
+  constexpr int PD = S - 1;  // rows of TD prefetched
   extern __shared__ uint16_t slots[];
-  __shared__ int32_t xtot[W];  // per-wave scan totals
-  __shared__ int32_t xbnd[W];  // per-wave last-column F value of the row
+  // Per-wave scan totals, double-buffered by row parity: a wave may write
+  // row r-1's total while a slower wave still reads row r's.
+  __shared__ int32_t xtot[2][W];
 
   const int64_t tix = blockIdx.x;
   if (tix >= A.ntasks) return;
@@ -397,9 +400,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
 
         int32_t carry = (rmax - r) + 1;  // F[r][cmax] - (-1)
         if constexpr (W > 1) {
-          if (lane == 0) xtot[wave] = carry_w;
+          int32_t* xt = xtot[r & 1];
+          if (lane == 0) xt[wave] = carry_w;
           __syncthreads();
-          for (int w = 0; w < wave; ++w) carry = min(carry, xtot[w]);
+          for (int w = 0; w < wave; ++w) carry = min(carry, xt[w]);
         }
         // h becomes G - t; Fpm = G + 1 - t = h + 1.
 #pragma unroll
@@ -430,14 +434,17 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           for (int j = 0; j < C; ++j)
             if (woff[j] >= 0) wrow_ptr[woff[j]] = (uint16_t)(h[j] - negt[j]);
           if (threadIdx.x == 0) wrow_ptr[width] = (uint16_t)(rmax - r);
+          // The next row may read this slot row at columns of other waves.
+          // (Every other LDS reuse is ordered by the barrier above: a slot
+          // row is only rewritten after the rows that read it.)
+          if constexpr (W > 1) __syncthreads();
         }
 
-        int32_t left_in = rmax - r;  // F[r][cmax]
-        if constexpr (W > 1) {
-          if (lane == 63) xbnd[wave] = h[C - 1] - negt[C - 1];
-          __syncthreads();
-          if (wave > 0) left_in = xbnd[wave - 1];
-        }
+        // F[r][c] at the column just before this wave's block (t = w*64*C-1).
+        // The previous wave's last element ends with h = min(its carry-in,
+        // its own total) = this wave's carry-in, and F = h + t; for wave 0
+        // this gives F[r][cmax] = rmax - r.
+        const int32_t left_in = carry + (wave * 64 * C - 1);
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           const int jl = g * E + E - 1;  // the group's last column

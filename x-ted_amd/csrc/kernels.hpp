@@ -45,7 +45,7 @@ struct PlanArgs {
   unsigned long long* counts;  // nlevels*kNumClasses
   unsigned long long* cells;   // nlevels*kNumClasses: interior cells per bucket
   unsigned long long* cursor;  // scatter cursors (bucket offsets)
-  int32_t* maxlds;             // kNumClasses
+  int32_t* maxlds;             // nlevels*kNumClasses: largest slot storage per bucket
   uint64_t* tasks;
 };
 // Blocks a pair occupies in the plan kernels (same formula on both sides).

@@ -110,7 +110,9 @@ __device__ __forceinline__ unsigned long long wave_reserve(unsigned long long* c
 // Count pass: per distinct bucket of the wave, one fire-and-forget atomic
 // each for the table count, the interior cells (a table has < 2^30 cells;
 // the wave sum is taken over 16-bit halves so 32-bit sums cannot overflow)
-// and the largest slot-storage need of the bucket's class.
+// and the largest slot-storage need of the bucket. All three arrays are
+// indexed by bucket, so the atomics of different buckets fall on different
+// cache lines (a per-class maximum put every wave of the chip on one line).
 __device__ __forceinline__ void wave_count(const PlanArgs& A, int bucket, bool active,
                                            uint32_t cells, int32_t lds) {
   unsigned long long todo = __ballot(active);
@@ -126,7 +128,7 @@ __device__ __forceinline__ void wave_count(const PlanArgs& A, int bucket, bool a
     if (lane == leader) {
       atomicAdd(&A.counts[lb], (unsigned long long)__popcll(same));
       atomicAdd(&A.cells[lb], ((unsigned long long)hi << 16) + lo);
-      if (mx > 0) atomicMax(&A.maxlds[lb % kNumClasses], mx);
+      if (mx > 0) atomicMax(&A.maxlds[lb], mx);
     }
     todo &= ~same;
   }
@@ -198,8 +200,8 @@ __global__ void plan_clear_kernel(unsigned long long* counts, unsigned long long
   if (i < nb) {
     counts[i] = 0;
     cells[i] = 0;
+    maxlds[i] = 0;
   }
-  if (i < kNumClasses) maxlds[i] = 0;
 }
 
 void launch_plan(const PlanArgs& a, int64_t nblocks, hipStream_t s) {
