@@ -21,6 +21,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -35,6 +36,7 @@ import xted  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 4
 CONFIG_BATCH = 3
+CONFIG_SERVE = 5
 
 
 def batch_trees(npairs, cfg=CONFIG_BATCH):
@@ -46,6 +48,71 @@ def batch_trees(npairs, cfg=CONFIG_BATCH):
         kind = xted.GEN_DEEP if (s >> 20) & 1 else xted.GEN_SHALLOW
         trees.append(xted.generate(kind, int(n), 200, s))
     return trees
+
+
+def serving_trees(ntrees, cfg=CONFIG_SERVE):
+    """Pool for serving: n log-uniform in [50, 5000], deep/shallow 50/50,
+    alphabet 200, seeded per tree."""
+    trees = []
+    for t in range(ntrees):
+        s = xted.derive_seed(cfg, t)
+        u = (s & 0xFFFFFFFF) / 2.0**32
+        n = int(round(math.exp(math.log(50) + u * (math.log(5000) - math.log(50)))))
+        kind = xted.GEN_DEEP if (s >> 40) & 1 else xted.GEN_SHALLOW
+        trees.append(xted.generate(kind, n, 200, s))
+    return trees
+
+
+def serve_bench(trees, seconds, rate, budget, max_batch, seed=7):
+    """Poisson arrivals of uniformly random pairs at `rate` per second for
+    `seconds`, through the continuous-batching server. Returns the latency
+    percentiles (submit -> result, measured by the server), the completed
+    pairs/s and a check of a sample of results against one-pair batches."""
+    rng = np.random.default_rng(seed)
+    n_arr = max(1, int(rng.poisson(rate * seconds)))
+    at = np.sort(rng.uniform(0.0, seconds, n_arr))
+    qx = rng.integers(0, len(trees), n_arr)
+    qy = rng.integers(0, len(trees), n_arr)
+    ctx = xted.Context(0)
+    t = time.perf_counter()
+    ids = ctx.pool_add(trees)
+    upload_s = time.perf_counter() - t
+    srv = xted.Server(ctx, hbm_budget=budget, max_batch=max_batch)
+    res = {}
+    t0 = time.perf_counter()
+    k = 0
+    while len(res) < n_arr:
+        now = time.perf_counter() - t0
+        while k < n_arr and at[k] <= now:
+            srv.submit(int(ids[qx[k]]), int(ids[qy[k]]), k)
+            k += 1
+        wait_ms = 1 if k < n_arr else 20
+        for tag, status, dist, lat, queue in srv.poll(cap=4096, timeout_ms=wait_ms):
+            res[tag] = (status, dist, lat, queue, time.perf_counter() - t0)
+        if time.perf_counter() - t0 > seconds + 120:
+            break
+    srv.close()
+    ok = [v for v in res.values() if v[0] == 0]
+    lat = np.array([v[2] for v in ok])
+    span = max(v[4] for v in res.values()) if res else seconds
+    # Spot check: a sample of served results against one-pair batches.
+    sample = sorted(res)[:: max(1, len(res) // 16)]
+    for tag in sample:
+        if res[tag][0] == 0:
+            d = ctx.batch(np.array([ids[qx[tag]]]), np.array([ids[qy[tag]]]))[0][0]
+            if d != res[tag][1]:
+                raise SystemExit(f"served distance differs from the one-pair batch (query {tag})")
+    ctx.close()
+    return {
+        "offered_rate": rate, "seconds": seconds, "queries": n_arr,
+        "completed": len(ok), "rejected": sum(1 for v in res.values() if v[0] != 0),
+        "pairs_per_s": round(len(ok) / span, 1),
+        "p50_ms": round(float(np.percentile(lat, 50)), 2) if len(lat) else None,
+        "p99_ms": round(float(np.percentile(lat, 99)), 2) if len(lat) else None,
+        "mean_queue_ms": round(float(np.mean([v[3] for v in ok])), 2) if ok else None,
+        "hbm_budget": budget, "max_batch": max_batch, "pool_trees": len(trees),
+        "pool_upload_s": round(upload_s, 2), "checked_against_single": len(sample),
+    }
 
 
 def log(msg):
@@ -75,6 +142,35 @@ def cpu_baseline(trees, budget_s):
                       f"(X-TED_CPU, {threads} threads, {dt:.1f} s)"}
 
 
+def copy_calibration(mib):
+    """A plain copy of known size: rocprofv3 --pmc FETCH_SIZE (or WRITE_SIZE)
+    on it gives the counter's KiB per copied KiB on this device."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    assert torch.equal(dst[-4:], src[-4:])
+    print(json.dumps({"copy_bytes": 4 * n, "copies": 3}), flush=True)
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    profiles/traffic_<kernel>.txt (FETCH_SIZE + WRITE_SIZE per dispatch,
+    scaled by the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,7 +182,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the granularity rule first and use the fastest")
+    ap.add_argument("--serve-seconds", type=float, default=0.0,
+                    help="also run the serving leg (Poisson arrivals) for this long")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-trees", type=int, default=10000)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (the "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(pairs would shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
@@ -133,7 +239,8 @@ def main():
     cls, (launches, tables, cells, ms) = dom
     achieved = BYTES_PER_CELL * cells / (ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profile(xted.class_name(cls)),
                 "kernel": xted.class_name(cls), "launches": launches,
                 "avg_launch_ms": round(ms / launches, 4),
                 "bytes_per_launch": int(BYTES_PER_CELL * cells / launches)}
@@ -148,11 +255,20 @@ def main():
         if dp != dist[i]:
             raise SystemExit(f"plain path distance differs from batch on pair {i}")
     plain_s = time.perf_counter() - t
-    plain_pairs_per_s = n_plain / plain_s
+    plain_pairs_per_s = n_plain / plain_s if n_plain else None
     log(f"plain: {n_plain} pairs in {plain_s:.2f} s")
     ctx.close()
 
     cpu = None if args.no_cpu else cpu_baseline(trees, args.cpu_budget)
+
+    serving = None
+    if args.serve_seconds > 0:
+        t = time.perf_counter()
+        strees = serving_trees(args.serve_trees)
+        log(f"serving pool: {len(strees)} trees in {time.perf_counter() - t:.1f} s")
+        serving = serve_bench(strees, args.serve_seconds, args.serve_rate,
+                              budget=32 << 30, max_batch=4096)
+        log(f"serving: {serving}")
 
     out = {
         "metric": "TED pairs/s (4096-pair uneven batch, 100-1000 nodes, batched persistent "
@@ -174,12 +290,13 @@ def main():
         "gcups": round(gcups, 2),
         "plan_ms": round(st.plan_ms, 3),
         "dp_ms": round(st.dp_ms, 3),
-        "plain_pairs_per_s": round(plain_pairs_per_s, 2),
+        "plain_pairs_per_s": round(plain_pairs_per_s, 2) if n_plain else None,
         "plain_sample_pairs": n_plain,
-        "batched_vs_plain": round(pairs_per_s / plain_pairs_per_s, 2),
+        "batched_vs_plain": round(pairs_per_s / plain_pairs_per_s, 2) if n_plain else None,
         "roofline": roofline,
         "classes": classes,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

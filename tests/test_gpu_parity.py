@@ -214,3 +214,68 @@ def test_granularity_rules_do_not_change_results(gpu_ctx, rule):
     for k in range(0, 24, 6):
         od, otd, _ = oracle.ted(trees[2 * k], trees[2 * k + 1], threads=4)
         assert got[0][k] == od and np.array_equal(got[1][k], otd)
+
+
+@pytest.mark.parametrize("cls", [27, 28, 29, 30, 31])
+def test_every_column_blocked_form_forced(gpu_ctx, cls):
+    """Every multi-row table goes through one column-blocked form (blocks of
+    4096 .. 256 columns, one workgroup each, chained per row through device
+    memory). With 256-column blocks a 1200-column table spans 5 blocks, so
+    carries and imported far values cross several block boundaries."""
+    gpu_ctx.force_class(cls)
+    try:
+        for i, (kx, ky, n1, n2) in enumerate([(0, 1, 300, 420), (2, 0, 650, 500),
+                                              (1, 2, 900, 1200), (1, 1, 700, 650)]):
+            x = gen(kx, n1, 12, 50 + cls, 2 * i)
+            y = gen(ky, n2, 12, 50 + cls, 2 * i + 1)
+            assert_same(gpu_ctx, x, y, threads=16)
+    finally:
+        gpu_ctx.force_class(xted.CLASS_AUTO)
+
+
+def test_column_blocked_batch_and_streams(gpu_ctx):
+    """Forced column-blocked tables in a batch, launches spread over streams."""
+    gpu_ctx.force_class(30)
+    try:
+        trees, xs, ys, shapes = _mixed_batch(gpu_ctx, 12, 3)
+        dist, tds, _ = gpu_ctx.batch(xs, ys, td_shapes=shapes)
+    finally:
+        gpu_ctx.force_class(xted.CLASS_AUTO)
+    for k in range(12):
+        od, otd, _ = oracle.ted(trees[2 * k], trees[2 * k + 1], threads=8)
+        assert dist[k] == od and np.array_equal(tds[k], otd)
+
+
+def _xl_case(name):
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_xl_golden",
+                                                  os.path.join(GOLDEN, "make_xl_golden.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    with open(os.path.join(GOLDEN, "xl_expected.json")) as f:
+        table = json.load(f)
+    if name not in table:
+        pytest.skip(f"no oracle checksum for {name} in xl_expected.json")
+    want = table[name]
+    return mod, mod.trees_of(name), want
+
+
+@pytest.mark.parametrize("name", ["rrt9000_a500", "deep5000_a200", "rrt20000_a500"])
+def test_large_pairs_against_oracle_checksums(gpu_ctx, name):
+    """Pairs whose largest tables need the column-blocked form by width
+    (RRT 9000 and 20000: more than 8192 columns) or by slot storage (deep
+    5000: nesting height above 100): distance and two checksums of the full
+    TD against the oracle's (tests/golden/make_xl_golden.py)."""
+    mod, (x, y), want = _xl_case(name)
+    d, td, st = gpu_ctx.ted(x, y)
+    assert d == want["dist"]
+    assert mod.td_checksums(td) == (want["td_sum"], want["td_wsum"])
+
+
+def test_large_pair_transposed(gpu_ctx):
+    """ted(y, x) gives the transposed table of ted(x, y) at full size."""
+    _, (x, y), want = _xl_case("rrt9000_a500")
+    d1, t1, _ = gpu_ctx.ted(x, y)
+    d2, t2, _ = gpu_ctx.ted(y, x)
+    assert d1 == d2 == want["dist"]
+    assert np.array_equal(t1, t2.T)

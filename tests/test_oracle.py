@@ -278,3 +278,26 @@ def test_planted_mistake_is_caught():
             caught = True
         assert d == oracle.brute_force_ted(x, y)
     assert caught
+
+
+def test_leaf_rows_closed_form():
+    """TED({v}, T) = |T| - [label(v) occurs in T]: every leaf row and leaf
+    column of the oracle's TD (the cells the leaf kernel writes in closed
+    form) must follow it."""
+    import xted
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        x = xted.generate(int(rng.integers(0, 3)), int(rng.integers(1, 70)), 4,
+                          int(rng.integers(1 << 30)))
+        y = xted.generate(int(rng.integers(0, 3)), int(rng.integers(1, 70)), 4,
+                          int(rng.integers(1 << 30)))
+        _, td, _ = oracle.ted(x, y, threads=2)
+        ox, oy = np.asarray(oracle.orl(x[0])), np.asarray(oracle.orl(y[0]))
+        for i in np.flatnonzero(ox == np.arange(len(ox))):
+            for c in range(len(oy)):
+                want = oy[c] - c + 1 - int(x[1][i] in y[1][c:oy[c] + 1])
+                assert td[i, c] == want
+        for j in np.flatnonzero(oy == np.arange(len(oy))):
+            for r in range(len(ox)):
+                want = ox[r] - r + 1 - int(y[1][j] in x[1][r:ox[r] + 1])
+                assert td[r, j] == want

@@ -26,8 +26,8 @@ import xted  # noqa: E402
 
 DEFAULT = {"lds_per_wave": 10 * 1024, "max_cpl": 8, "min_waves": 1, "wide_cols": 1 << 30}
 GRID = {
-    "lds_per_wave": [10 * 1024, 20 * 1024, 40 * 1024, 160 * 1024],
-    "max_cpl": [4, 8, 16],
+    "lds_per_wave": [5 * 1024, 10 * 1024, 20 * 1024, 40 * 1024],
+    "max_cpl": [2, 4, 8],
     "min_waves": [1],
     "wide_cols": [1 << 30],
 }

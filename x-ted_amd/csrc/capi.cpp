@@ -204,7 +204,7 @@ int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t max_cpl,
 }
 
 int xted_force_class(xted_context* ctx, int32_t cls, int32_t lane_enable) {
-  if (!ctx || cls < -1 || cls > kNumRowClasses) {
+  if (!ctx || cls < -1 || cls == kClassXL || cls >= kNumClasses) {
     set_error("xted_force_class: invalid class");
     return ERR_INVALID_ARG;
   }

@@ -59,6 +59,7 @@ struct PoolView {
   const int32_t* sleaf;
   const int32_t* spre;   // prefix sums of ssize (K+1 entries)
   const int32_t* kdoff;
+  const int32_t* rtop;   // keyroot at the top of node's right path
   const TreeMeta* trees;
 };
 
@@ -74,14 +75,27 @@ XTED_HD inline uint32_t task_i0(uint64_t t) { return (uint32_t)(t >> 20) & 0xFFF
 XTED_HD inline uint32_t task_j0(uint64_t t) { return (uint32_t)t & 0xFFFFFu; }
 
 // ---- granularity classes ----------------------------------------------------
-// Class 0: lane-per-table (a table whose short side is one node: one row).
+// Class 0: leaf (a table whose short side is one node, a leaf keyroot),
+//   evaluated in closed form by the leaf kernel, one line per leaf keyroot.
 // Classes 1..25: workgroup row scan with W = 2^wi waves and C = 2^ci columns
 //   per lane, class = 1 + wi*5 + ci (wi, ci in 0..4).
-// Class 26: tables too wide or too LDS-hungry for the on-chip form.
-constexpr int kClassLane = 0;
+// Class 26: tables no form below can hold (an error).
+// Classes 27..31: column-blocked form for tables too wide or too LDS-hungry
+//   for one workgroup: blocks of xl_bw(k) columns, one workgroup each,
+//   chained per row through device memory, class = 27 + k.
+constexpr int kClassLane = 0;  // the leaf class
 constexpr int kNumRowClasses = 25;
 constexpr int kClassXL = 26;
-constexpr int kNumClasses = 27;
+constexpr int kFirstXLClass = 27;
+constexpr int kNumXLForms = 5;
+constexpr int kNumClasses = kFirstXLClass + kNumXLForms;  // 32
+// Column-blocked forms: waves and columns per lane of one block.
+XTED_HD inline int xl_waves(int k) { return k <= 1 ? 8 : k == 2 ? 4 : 2; }
+XTED_HD inline int xl_cpl(int k) { return k == 0 ? 8 : k <= 3 ? 4 : 2; }
+XTED_HD inline int xl_bw(int k) { return 64 * xl_waves(k) * xl_cpl(k); }  // 4096 .. 256
+// Far values a block takes from earlier blocks are indexed by the nesting
+// depth of their right path below the column keyroot: at most this many.
+constexpr int kMaxImports = 256;
 XTED_HD inline int row_class(int wi, int ci) { return 1 + wi * 5 + ci; }
 
 // Saved-row slots are uint16 (F <= |subtree x| + |subtree y| < 65536).
@@ -106,12 +120,36 @@ XTED_HD inline int ilog2_ceil(int v) {
   return r;
 }
 
-// Class of table (a x b) given the LDS need of its oriented form.
-XTED_HD inline int table_class(int a, int b, int64_t lds_need, const ClassRule& rule) {
+// Slot bytes of one block of a column-blocked table: one uint16 row per
+// nesting level of the row keyroot, each holding the block's own far columns
+// (at most min(leaves, bw)), the boundary entry and kd_c+1 imports.
+XTED_HD inline int64_t xl_lds_need(int kd_r, int kd_c, int leaves_c, int bw) {
+  const int own = leaves_c < bw ? leaves_c : bw;
+  return (int64_t)(kd_r > 1 ? kd_r : 1) * (own + 2 + kd_c) * 2;
+}
+
+// Class of table (a x b): lds_need is the slot storage of its oriented
+// on-chip form; kd_r / kd_c / leaves_c (row keyroot height, column keyroot
+// height and leaves) size the column-blocked forms. Single-row tables are
+// the leaf class when leaf_form is set. *need receives the slot bytes of the
+// chosen form.
+XTED_HD inline int table_class(int a, int b, int64_t lds_need, int kd_r, int kd_c, int leaves_c,
+                               bool leaf_form, const ClassRule& rule, int64_t* need) {
   const int rows = a <= b ? a : b;
   const int cols = a <= b ? b : a;
-  if (rows == 1) return kClassLane;
-  if (cols > kMaxCols || lds_need > rule.lds_max) return kClassXL;
+  *need = lds_need;
+  if (rows == 1 && leaf_form) return kClassLane;
+  if (cols > 64 * rule.max_cpl * kMaxRowWaves || lds_need > rule.lds_max) {
+    if (kd_c + 1 > kMaxImports) return kClassXL;
+    for (int k = 0; k < kNumXLForms; ++k) {
+      const int64_t n = xl_lds_need(kd_r, kd_c, leaves_c, xl_bw(k));
+      if (n <= rule.lds_max) {
+        *need = n;
+        return kFirstXLClass + k;
+      }
+    }
+    return kClassXL;
+  }
   int wi = 0;
   while (wi < 4 && lds_need > (int64_t)rule.lds_per_wave * (1 << wi)) ++wi;
   while (wi < 4 && cols > 64 * rule.max_cpl * (1 << wi)) ++wi;
@@ -121,7 +159,6 @@ XTED_HD inline int table_class(int a, int b, int64_t lds_need, const ClassRule& 
   int w = 1 << wi;
   int c = (cols + 64 * w - 1) / (64 * w);
   int ci = ilog2_ceil(c);
-  if (ci > 4) return kClassXL;
   return row_class(wi, ci);
 }
 

@@ -79,7 +79,8 @@ Context::~Context() {
   for (DevBuf* b : {&parent_, &label_, &orl_, &kdep_, &kdn_, &lrank_, &kr_, &krkd_, &krsize_,
                     &krleaf_, &trees_, &prep_scratch_, &prep_off_, &pairs_, &blk_, &counts_,
                     &cells_, &cursor_, &maxlds_, &tasks_, &td_, &dist_, &skr_, &skd_, &ssize_,
-                    &sleaf_, &spre_, &kdoff_, &lane_off_, &lane_tot_, &lane_cells_})
+                    &sleaf_, &spre_, &kdoff_, &rtop_, &lines_, &tickets_, &abort_, &xl_off_,
+                    &xl_total_, &xl_words_})
     b->release();
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
@@ -138,6 +139,7 @@ PoolView Context::view() const {
   v.sleaf = sleaf_.as<int32_t>();
   v.spre = spre_.as<int32_t>();
   v.kdoff = kdoff_.as<int32_t>();
+  v.rtop = rtop_.as<int32_t>();
   v.trees = trees_.as<TreeMeta>();
   return v;
 }
@@ -145,7 +147,7 @@ PoolView Context::view() const {
 int Context::grow_pool(int64_t nodes) {
   const size_t bytes = (size_t)nodes * sizeof(int32_t);
   for (DevBuf* b : {&parent_, &label_, &orl_, &kdep_, &kdn_, &lrank_, &kr_, &krkd_, &krsize_,
-                    &krleaf_, &skr_, &skd_, &ssize_, &sleaf_, &spre_, &kdoff_})
+                    &krleaf_, &skr_, &skd_, &ssize_, &sleaf_, &spre_, &kdoff_, &rtop_})
     ST_TRY(b->grow(bytes, true, stream_));
   return OK;
 }
@@ -238,6 +240,7 @@ int Context::pool_add(int32_t ntrees, const int32_t* sizes, const int32_t* paren
   pa.sleaf = sleaf_.as<int32_t>();
   pa.spre = spre_.as<int32_t>();
   pa.kdoff = kdoff_.as<int32_t>();
+  pa.rtop = rtop_.as<int32_t>();
   pa.trees = trees_.as<TreeMeta>();
   pa.scratch = prep_scratch_.as<int32_t>();
   pa.scratch_off = prep_off_.as<int64_t>();
@@ -297,9 +300,9 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   }
   HIP_TRY(hipSetDevice(device_));
   std::vector<PairMeta> pm(npairs);
-  std::vector<int64_t> blk(npairs + 1, 0);
-  int64_t td_total = 0, planned = 0;
-  int32_t nlevels = 0;
+  std::vector<int64_t> blk(npairs + 1, 0), lines(npairs + 1, 0);
+  int64_t td_total = 0, planned = 0, leaf_tables = 0, leaf_cells = 0;
+  int32_t nlevels = 0, maxn = 1;
   for (int32_t i = 0; i < npairs; ++i) {
     const int32_t x = xs[i], y = ys[i];
     if (x < 0 || y < 0 || x >= (int32_t)meta_.size() || y >= (int32_t)meta_.size()) {
@@ -308,10 +311,17 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     }
     const TreeMeta &a = meta_[x], &b = meta_[y];
     const int64_t mn = (int64_t)a.n * b.n;
+    maxn = std::max(maxn, std::max(a.n, b.n));
     pm[i] = PairMeta{x, y, td_total + kTdPad, td_total + 2 * kTdPad + mn};
     td_total += 2 * mn + 3 * kTdPad;
-    // With the lane kernel on, the plan covers only non-leaf keyroot pairs.
+    // With the leaf kernel on, the plan covers only non-leaf keyroot pairs;
+    // the leaf kernel writes one line per leaf keyroot of either tree.
     const int32_t ka = lane_enable ? a.K - a.K1 : a.K, kb = lane_enable ? b.K - b.K1 : b.K;
+    lines[i + 1] = lines[i] + (lane_enable ? a.K1 + b.K1 : 0);
+    if (lane_enable) {
+      leaf_tables += (int64_t)a.K1 * b.K + (int64_t)(a.K - a.K1) * b.K1;
+      leaf_cells += (int64_t)a.K1 * info_[y].ksum + (int64_t)b.K1 * (info_[x].ksum - a.K1);
+    }
     blk[i + 1] = blk[i] + plan_blocks(ka, kb, plan_chunk);
     planned += (int64_t)ka * kb;
     nlevels = std::max(nlevels, a.kdmax + b.kdmax + 1);
@@ -325,15 +335,17 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   ST_TRY(cells_.grow(sizeof(unsigned long long) * nb, false, stream_));
   ST_TRY(cursor_.grow(sizeof(unsigned long long) * nb, false, stream_));
   ST_TRY(maxlds_.grow(sizeof(int32_t) * nb, false, stream_));
+  ST_TRY(tickets_.grow(sizeof(unsigned int) * nb, false, stream_));
+  ST_TRY(abort_.grow(sizeof(int32_t), false, stream_));
   ST_TRY(tasks_.grow(sizeof(uint64_t) * std::max<int64_t>(planned, 1), false, stream_));
   ST_TRY(td_.grow(sizeof(int32_t) * td_total, false, stream_));
   ST_TRY(dist_.grow(sizeof(int32_t) * npairs, false, stream_));
-  ST_TRY(lane_off_.grow(sizeof(unsigned long long) * nlevels * (npairs + 1), false, stream_));
-  ST_TRY(lane_tot_.grow(sizeof(unsigned long long) * nlevels, false, stream_));
-  ST_TRY(lane_cells_.grow(sizeof(unsigned long long) * nlevels, false, stream_));
+  ST_TRY(lines_.grow(sizeof(int64_t) * (npairs + 1), false, stream_));
   HIP_TRY(hipMemcpyAsync(pairs_.p, pm.data(), sizeof(PairMeta) * npairs, hipMemcpyHostToDevice,
                          stream_));
   HIP_TRY(hipMemcpyAsync(blk_.p, blk.data(), sizeof(int64_t) * (npairs + 1),
+                         hipMemcpyHostToDevice, stream_));
+  HIP_TRY(hipMemcpyAsync(lines_.p, lines.data(), sizeof(int64_t) * (npairs + 1),
                          hipMemcpyHostToDevice, stream_));
   HIP_TRY(hipEventRecord(ev_[0], stream_));
 
@@ -352,27 +364,16 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   pa.cursor = cursor_.as<unsigned long long>();
   pa.maxlds = maxlds_.as<int32_t>();
   pa.tasks = tasks_.as<uint64_t>();
+  pa.tickets = tickets_.as<unsigned int>();
+  pa.abort_flag = abort_.as<int32_t>();
   launch_plan(pa, blk[npairs], stream_);
   HIP_TRY(hipGetLastError());
-  LaneArgs la;
+  LeafArgs la;
   la.pool = pa.pool;
   la.pairs = pa.pairs;
+  la.line_start = lines_.as<int64_t>();
   la.npairs = npairs;
-  la.level = 0;
-  la.offsets = lane_off_.as<unsigned long long>();
-  la.totals = lane_tot_.as<unsigned long long>();
-  la.cells = lane_cells_.as<unsigned long long>();
   la.td = td_.as<int32_t>();
-  h_lane_tot_.assign(nlevels, 0);
-  h_lane_cells_.assign(nlevels, 0);
-  if (lane_enable) {
-    launch_lane_offsets(la, nlevels, stream_);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(h_lane_tot_.data(), lane_tot_.p, sizeof(unsigned long long) * nlevels,
-                           hipMemcpyDeviceToHost, stream_));
-    HIP_TRY(hipMemcpyAsync(h_lane_cells_.data(), lane_cells_.p,
-                           sizeof(unsigned long long) * nlevels, hipMemcpyDeviceToHost, stream_));
-  }
   h_counts_.assign(nb, 0);
   h_maxlds_.assign(nb, 0);
   HIP_TRY(hipMemcpyAsync(h_counts_.data(), counts_.p, sizeof(unsigned long long) * nb,
@@ -384,24 +385,60 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
                          hipMemcpyDeviceToHost, stream_));
   HIP_TRY(hipStreamSynchronize(stream_));
   HIP_TRY(hipEventRecord(ev_[1], stream_));
+  // Column-blocked tables: ordinal of each bucket's first task, then (with a
+  // second short sync, only in batches that have such tables) the arena
+  // layout computed on the device.
+  std::vector<int64_t> xl_first(nb, 0);
+  int64_t nxl = 0;
   for (int L = 0; L < nlevels; ++L) {
     if (h_counts_[(size_t)L * kNumClasses + kClassXL]) {
-      set_error("run_batch: a table exceeds the on-chip form (more than " +
-                std::to_string(kMaxCols) + " columns or " + std::to_string(lds_limit) +
-                " bytes of slot storage)");
+      set_error("run_batch: a table fits no form (nesting height above " +
+                std::to_string(kMaxImports - 1) + " or slot storage above " +
+                std::to_string(lds_limit) + " bytes even in 256-column blocks)");
       return ERR_TOO_LARGE;
+    }
+    for (int k = 0; k < kNumXLForms; ++k) {
+      const size_t bkt = (size_t)L * kNumClasses + kFirstXLClass + k;
+      xl_first[bkt] = nxl;
+      nxl += (int64_t)h_counts_[bkt];
+    }
+  }
+  if (nxl > 0) {
+    ST_TRY(xl_off_.grow(sizeof(int64_t) * nxl, false, stream_));
+    ST_TRY(xl_total_.grow(sizeof(unsigned long long), false, stream_));
+    XLLayoutArgs xa;
+    xa.pool = pa.pool;
+    xa.pairs = pa.pairs;
+    xa.tasks = pa.tasks;
+    xa.counts = pa.counts;
+    xa.cursor = pa.cursor;
+    xa.nlevels = nlevels;
+    xa.off = xl_off_.as<int64_t>();
+    xa.total = xl_total_.as<unsigned long long>();
+    launch_xl_layout(xa, stream_);
+    HIP_TRY(hipGetLastError());
+    unsigned long long words = 0;
+    HIP_TRY(hipMemcpyAsync(&words, xa.total, sizeof(words), hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    const size_t bytes = sizeof(unsigned long long) * std::max<unsigned long long>(words, 1);
+    if (bytes > xl_words_.cap) {
+      ST_TRY(xl_words_.grow(bytes, false, stream_));
+      launch_xl_zero(xl_words_.as<unsigned long long>(), (int64_t)(xl_words_.cap / 8), stream_);
+      HIP_TRY(hipGetLastError());
     }
   }
   DPArgs da;
   da.pool = pa.pool;
   da.pairs = pa.pairs;
   da.td = td_.as<int32_t>();
+  da.xl_words = xl_words_.as<unsigned long long>();
+  da.xl_abort = abort_.as<int32_t>();
   // With profiling on, an event pair brackets every launch (same stream).
   std::vector<std::pair<int, int>> prof;  // (class, bucket or -1-level) per timed launch
   if (profile) {
     size_t need = 0;
     for (unsigned long long c : h_counts_) need += c ? 2 : 0;
-    for (unsigned long long c : h_lane_tot_) need += c ? 2 : 0;
+    need += 2;
     while (prof_ev_.size() < need) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
@@ -420,7 +457,11 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   unsigned long long off = 0;
   for (int L = 0; L < nlevels; ++L) {
     lv.clear();
-    if (h_lane_tot_[L]) lv.push_back({kClassLane, -1 - L, 0, h_lane_tot_[L], h_lane_cells_[L]});
+    // The leaf lines read nothing and level-0 tables (two chains) read no TD,
+    // so the leaf kernel runs beside level 0.
+    if (L == 0 && lines[npairs] > 0)
+      lv.push_back({kClassLane, -1, 0, (unsigned long long)lines[npairs],
+                    (unsigned long long)leaf_cells});
     for (int c = 0; c < kNumClasses; ++c) {
       const size_t b = (size_t)L * kNumClasses + c;
       if (h_counts_[b]) lv.push_back({c, (int)b, off, h_counts_[b], h_cells_[b]});
@@ -436,11 +477,17 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
       hipStream_t s = (i % ns) == 0 ? stream_ : side_[i % ns - 1];
       if (profile) HIP_TRY(hipEventRecord(prof_ev_[2 * prof.size()], s));
       if (w.cls == kClassLane) {
-        la.level = L;
-        launch_lane_level(la, w.cnt, s);
+        launch_leaf(la, (int64_t)w.cnt, s);
       } else {
         da.tasks = tasks_.as<uint64_t>() + w.off;
         da.ntasks = (int64_t)w.cnt;
+        if (w.cls >= kFirstXLClass) {
+          const int bw = xl_bw(w.cls - kFirstXLClass);
+          da.xl_off = xl_off_.as<int64_t>() + xl_first[w.bucket];
+          da.xl_ticket = tickets_.as<unsigned int>() + w.bucket;
+          da.xl_epoch = ++xl_epoch_ ? xl_epoch_ : ++xl_epoch_;  // never 0
+          da.xl_nbmax = (maxn + bw - 1) / bw;
+        }
         const int lds = std::max(64, (h_maxlds_[w.bucket] + 15) & ~15);
         if (!launch_forest(w.cls, da, lds, s)) {
           set_error("run_batch: no kernel for class " + std::to_string(w.cls));
@@ -463,6 +510,9 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   launch_gather_dist(pa.pairs, da.td, npairs, dist_.as<int32_t>(), stream_);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(dist, dist_.p, sizeof(int32_t) * npairs, hipMemcpyDeviceToHost, stream_));
+  int32_t aborted = 0;
+  if (nxl > 0)
+    HIP_TRY(hipMemcpyAsync(&aborted, abort_.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
   if (td_out) {
     for (int32_t i = 0; i < npairs; ++i) {
       if (!td_out[i]) continue;
@@ -472,6 +522,10 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     }
   }
   HIP_TRY(hipStreamSynchronize(stream_));
+  if (aborted) {
+    set_error("run_batch: a column-blocked table stopped waiting for its neighbour block");
+    return ERR_TIMEOUT;
+  }
   if (profile) {
     for (auto& cp : class_profile) cp = ClassProfile{};
     for (size_t i = 0; i < prof.size(); ++i) {
@@ -481,8 +535,8 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
       cp.launches += 1;
       cp.ms += ms;
       const int b = prof[i].second;
-      cp.tables += (int64_t)(b >= 0 ? h_counts_[b] : h_lane_tot_[-1 - b]);
-      cp.cells += (int64_t)(b >= 0 ? h_cells_[b] : h_lane_cells_[-1 - b]);
+      cp.tables += b >= 0 ? (int64_t)h_counts_[b] : leaf_tables;
+      cp.cells += b >= 0 ? (int64_t)h_cells_[b] : leaf_cells;
     }
   }
   float plan_ms = 0, dp_ms = 0;

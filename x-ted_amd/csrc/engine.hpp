@@ -104,12 +104,16 @@ class Context {
   std::vector<HostTreeInfo> info_;
   int64_t pool_nodes_ = 0;  // used entries (sum of n+1)
   DevBuf parent_, label_, orl_, kdep_, kdn_, lrank_, kr_, krkd_, krsize_, krleaf_, trees_;
-  DevBuf skr_, skd_, ssize_, sleaf_, spre_, kdoff_;
+  DevBuf skr_, skd_, ssize_, sleaf_, spre_, kdoff_, rtop_;
   DevBuf prep_scratch_, prep_off_;
   // batch
   DevBuf pairs_, blk_, counts_, cells_, cursor_, maxlds_, tasks_, td_, dist_;
-  DevBuf lane_off_, lane_tot_, lane_cells_;
-  std::vector<unsigned long long> h_counts_, h_cells_, h_lane_tot_, h_lane_cells_;
+  DevBuf lines_;
+  // column-blocked tables: launch tickets, abort flag, per-task arena offsets,
+  // the arena of cross-workgroup words and the tag of the latest launch
+  DevBuf tickets_, abort_, xl_off_, xl_total_, xl_words_;
+  uint32_t xl_epoch_ = 0;
+  std::vector<unsigned long long> h_counts_, h_cells_;
   std::vector<int32_t> h_maxlds_;  // largest slot storage per (level, class) bucket
   std::vector<hipEvent_t> prof_ev_;
   int grow_pool(int64_t nodes);

@@ -59,132 +59,58 @@ __device__ __forceinline__ int wave_shr1(int v, int fill) {
 }
 
 // ---------------------------------------------------------------------------
-// Lane-per-table: tables whose short side is a single node. The row is the
-// single leaf r0 (so every far row is the boundary row), the lane walks the
-// columns of the other subtree right to left carrying F[r0][c+1].
-// Tasks with the same column keyroot sit next to each other, so the orl and
-// label loads of one wave are mostly uniform.
+// Leaf-keyroot lines in closed form. Line q of pair p is a leaf keyroot v of
+// one tree; the wave walks the other tree T right to left, 64 positions per
+// step, lane l at position c = hi - l. nxt(c), the first position >= c whose
+// label is label(v), is a prefix minimum over lanes plus the minimum carried
+// from the positions above, and TD = (orl(c) - c + 1) - [nxt(c) <= orl(c)].
+// The line is contiguous in one of TD / TD^T (coalesced) and strided in the
+// other.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void lane_table(const PoolView& P, const PairMeta& pm, int32_t i0,
-                                           int32_t j0, int32_t* td) {
-  const TreeMeta tx = P.trees[pm.tx], ty = P.trees[pm.ty];
-  const int32_t* orlx = P.orl + tx.node_off;
-  const int32_t* orly = P.orl + ty.node_off;
-  const int32_t a = orlx[i0] - i0 + 1, b = orly[j0] - j0 + 1;
-  const bool rows_x = a <= b;
-  const int64_t offR = rows_x ? tx.node_off : ty.node_off;
-  const int64_t offC = rows_x ? ty.node_off : tx.node_off;
-  const int32_t* labR = P.label + offR;
-  const int32_t* orlC = P.orl + offC;
-  const int32_t* labC = P.label + offC;
-  const int32_t r0 = rows_x ? i0 : j0, c0 = rows_x ? j0 : i0;
-  // TD in [r][c] and [c][r] layouts for this orientation.
-  int32_t* TDrc = td + (rows_x ? pm.td_off : pm.tdt_off);
-  int32_t* TDcr = td + (rows_x ? pm.tdt_off : pm.td_off);
-  const int64_t ldrc = rows_x ? ty.n : tx.n;
-  const int64_t ldcr = rows_x ? tx.n : ty.n;
-  const int32_t cmax = orlC[c0] + 1;
-  const int32_t orl_c0 = cmax - 1;
-  const int32_t labr = labR[r0];
-  // Loads run kAhead columns ahead of the recurrence: ring slot k holds
-  // column c-k's orl, label and TD value. The main loop (c - kAhead >= c0)
-  // walks pointers and never clamps; the last kAhead columns are already in
-  // the ring. Values are only combined when used, so no load waits inside a
-  // branch, and the main loop is unrolled by the ring depth so the rotation
-  // is register renaming.
-  constexpr int kAhead = 4;
-  int32_t ring_o[kAhead], ring_l[kAhead], ring_v[kAhead];
-#pragma unroll
-  for (int k = 0; k < kAhead; ++k) {
-    const int32_t c = max(cmax - 1 - k, c0);
-    ring_o[k] = orlC[c];
-    ring_l[k] = labC[c];
-    ring_v[k] = TDcr[(int64_t)c * ldcr + r0];
-  }
-  const int32_t* po = orlC + (cmax - 1 - kAhead);
-  const int32_t* pl = labC + (cmax - 1 - kAhead);
-  const int32_t* pv = TDcr + (int64_t)(cmax - 1 - kAhead) * ldcr + r0;
-  int32_t* pw_cr = TDcr + (int64_t)(cmax - 1) * ldcr + r0;
-  int32_t* pw_rc = TDrc + (int64_t)r0 * ldrc + (cmax - 1);
-  int32_t g = 1;  // F[r0][cmax] = imax - r0
-  int32_t t = 0;
-  auto step = [&]() {
-    const int32_t oc = ring_o[0], lab = ring_l[0], v = ring_v[0];
-#pragma unroll
-    for (int k = 0; k + 1 < kAhead; ++k) {
-      ring_o[k] = ring_o[k + 1];
-      ring_l[k] = ring_l[k + 1];
-      ring_v[k] = ring_v[k + 1];
-    }
-    const bool tree = (oc == orl_c0);
-    // tree cell: F[imax][c+1] + rename = t + (lab differ); forest cell:
-    // F[imax][orl(c)+1] + TD[r0][c] = (cmax - orl(c) - 1) + TD. Both terms
-    // are formed and one is selected, so the step has no divergent branch.
-    const int32_t dt = t + (int32_t)(labr != lab);
-    const int32_t df = (cmax - oc - 1) + v;
-    const int32_t d = tree ? dt : df;
-    g = min(min(d, t + 2), g + 1);
-    if (tree) {
-      *pw_rc = g;
-      *pw_cr = g;
-    }
-    pw_rc -= 1;
-    pw_cr -= ldcr;
-    ++t;
-  };
-  const int32_t ncols = cmax - c0;
-#pragma unroll 4
-  for (int32_t i = 0; i < ncols - kAhead; ++i) {
-    const int32_t o = *po, l = *pl, v = *pv;
-    step();
-    ring_o[kAhead - 1] = o;
-    ring_l[kAhead - 1] = l;
-    ring_v[kAhead - 1] = v;
-    po -= 1;
-    pl -= 1;
-    pv -= ldcr;
-  }
-#pragma unroll
-  for (int k = 0; k < kAhead; ++k)
-    if (t < ncols) step();
-}
-
-// Single-row tables of one level, without a task list: thread g finds its
-// pair in the level's prefix over pairs, then its cell of rectangle A
-// (consecutive lanes share the y keyroot) or B (consecutive lanes share the
-// x keyroot).
-__global__ __launch_bounds__(256) void lane_level_kernel(LaneArgs A, unsigned long long total) {
-  const unsigned long long g = (unsigned long long)blockIdx.x * 256 + threadIdx.x;
-  if (g >= total) return;
-  const unsigned long long* off = A.offsets + (int64_t)A.level * (A.npairs + 1);
+__global__ __launch_bounds__(256) void leaf_kernel(LeafArgs A, int64_t nlines) {
+  const int64_t line = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (line >= nlines) return;  // uniform per wave
+  const int lane = threadIdx.x & 63;
   int lo = 0, hi = A.npairs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (off[mid] <= g) lo = mid;
+    if (A.line_start[mid] <= line) lo = mid;
     else hi = mid - 1;
   }
-  const int p = lo;
-  int32_t sz[5];
-  (void)lane_rect_count(A, p, A.level, sz);
-  const PairMeta pm = A.pairs[p];
+  const PairMeta pm = A.pairs[lo];
   const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
-  int32_t q = (int32_t)(g - off[p]);
-  int32_t k, l;
-  if (q < sz[0]) {
-    k = q % tx.K1;
-    l = sz[3] + q / tx.K1;
-  } else {
-    q -= sz[0];
-    l = q % ty.K1;
-    k = sz[4] + q / ty.K1;
+  const int32_t q = (int32_t)(line - A.line_start[lo]);
+  const bool of_x = q < tx.K1;  // leaf keyroot of x: the line is TD[v][.]
+  const TreeMeta& R = of_x ? tx : ty;
+  const TreeMeta& T = of_x ? ty : tx;
+  const int32_t v = A.pool.skr[R.node_off + (of_x ? q : q - tx.K1)];
+  const int32_t lam = A.pool.label[R.node_off + v];
+  const int32_t* __restrict__ labT = A.pool.label + T.node_off;
+  const int32_t* __restrict__ orlT = A.pool.orl + T.node_off;
+  // of_x: TD[v][c] = td[td_off + v*ny + c], TD^T[c][v] = td[tdt_off + c*nx + v];
+  // else: TD^T[v][c] = td[tdt_off + v*nx + c], TD[c][v] = td[td_off + c*ny + v].
+  int32_t* pc = A.td + (of_x ? pm.td_off + (int64_t)v * ty.n : pm.tdt_off + (int64_t)v * tx.n);
+  int32_t* ps = A.td + (of_x ? pm.tdt_off : pm.td_off) + v;
+  const int64_t lds = of_x ? tx.n : ty.n;
+  int32_t carry = INT_MAX;
+  for (int32_t top = T.n - 1; top >= 0; top -= 64) {
+    const int32_t c = top - lane;
+    const int32_t cc = max(c, 0);
+    const int32_t lab = labT[cc], oc = orlT[cc];
+    const int32_t incl = wave_incl_min(c >= 0 && lab == lam ? c : INT_MAX);
+    const int32_t nxt = min(carry, incl);
+    carry = min(carry, __builtin_amdgcn_readlane(incl, 63));
+    const int32_t d = (oc - c + 1) - (int32_t)(nxt <= oc);
+    if (c >= 0) {
+      pc[c] = d;
+      ps[(int64_t)c * lds] = d;
+    }
   }
-  lane_table(A.pool, pm, A.pool.skr[tx.node_off + k], A.pool.skr[ty.node_off + l], A.td);
 }
 
-void launch_lane_level(const LaneArgs& a, unsigned long long total, hipStream_t s) {
-  if (total == 0) return;
-  hipLaunchKernelGGL(lane_level_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a,
-                     total);
+void launch_leaf(const LeafArgs& a, int64_t nlines, hipStream_t s) {
+  if (nlines <= 0) return;
+  hipLaunchKernelGGL(leaf_kernel, dim3((unsigned)((nlines + 3) / 4)), dim3(256), 0, s, a, nlines);
 }
 
 // ---------------------------------------------------------------------------
@@ -200,20 +126,117 @@ void launch_lane_level(const LaneArgs& a, unsigned long long total, hipStream_t 
 typedef int32_t int4u __attribute__((ext_vector_type(4), aligned(4)));
 typedef int32_t int2u __attribute__((ext_vector_type(2), aligned(4)));
 
-template <int W, int C>
+// ---------------------------------------------------------------------------
+// Column-blocked form (kXL). A table too wide or too LDS-hungry for one
+// workgroup is cut into blocks of BW = 64*W*C columns (t ranges), one
+// workgroup per block, every block sweeping all rows. Per row, block b needs
+// from the blocks before it
+//   * the scan carry min_{t' < b*BW} (a[t'] - t') (boundary included): block
+//     b-1 publishes min(its carry-in, its total); F at t = b*BW-1 is that
+//     carry + b*BW-1, so the left boundary and the next row's diagonal
+//     follow from it too;
+//   * far values F[R][f] of saved rows R at far columns f in earlier blocks.
+//     For a block those columns are the far columns of the right paths that
+//     cross its right edge, one per nesting depth below the column keyroot,
+//     so they live in kd_c+1 import entries of the block's slot rows. The
+//     owner of f writes them at its row R to every block up to the one
+//     holding the path's top keyroot.
+// Words crossing workgroups are {epoch:32 | value:32}, written once per
+// launch with device-scope relaxed atomics; a reader polls until the epoch
+// matches, so no fence is needed and nothing is cleared between launches.
+// Workgroups take (table, block) from a ticket counter in start order, so
+// a block only ever waits on blocks that are already running. Waits are
+// bounded: after kXLSpinLimit polls a workgroup raises the abort flag,
+// every waiting workgroup then stops waiting, and the call fails.
+// ---------------------------------------------------------------------------
+constexpr int kXLSpinLimit = 1 << 22;
+
+__device__ __forceinline__ unsigned long long xl_load(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xl_store(unsigned long long* p, uint32_t epoch, int32_t v) {
+  __hip_atomic_store(p, ((unsigned long long)epoch << 32) | (uint32_t)v, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+// Lanes with `need` poll *p until it carries `epoch` (w is the first read).
+__device__ __forceinline__ int32_t xl_wait(const unsigned long long* p, unsigned long long w,
+                                           bool need, uint32_t epoch, int32_t* abort_flag) {
+  for (int spins = 0; __ballot(need && (uint32_t)(w >> 32) != epoch); ++spins) {
+    if ((spins & 63) == 63) {
+      if (spins > kXLSpinLimit) __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+      if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    }
+    __builtin_amdgcn_s_sleep(2);
+    if (need && (uint32_t)(w >> 32) != epoch) w = xl_load(p);
+  }
+  return (int32_t)(uint32_t)w;
+}
+
+// Arena words of one column-blocked task: (nb-1)*nrows carries, then
+// nsaved*nb*kds export words (saved row, destination block, depth).
+struct XLGeom {
+  int32_t nrows, nb, nsaved, kds;
+};
+__device__ __forceinline__ XLGeom xl_geom(const PoolView& P, const PairMeta& pm, uint64_t task,
+                                          int bw) {
+  const TreeMeta tx = P.trees[pm.tx], ty = P.trees[pm.ty];
+  const int32_t i0 = (int32_t)task_i0(task), j0 = (int32_t)task_j0(task);
+  const int32_t a = P.orl[tx.node_off + i0] - i0 + 1, b = P.orl[ty.node_off + j0] - j0 + 1;
+  const bool rows_x = a <= b;
+  const int64_t offR = rows_x ? tx.node_off : ty.node_off;
+  const int64_t offC = rows_x ? ty.node_off : tx.node_off;
+  const int32_t r0 = rows_x ? i0 : j0, c0 = rows_x ? j0 : i0;
+  XLGeom g;
+  g.nrows = rows_x ? a : b;
+  g.nb = ((rows_x ? b : a) + bw - 1) / bw;
+  g.nsaved = P.lrank[offR + r0 + g.nrows] - P.lrank[offR + r0];
+  g.kds = P.kdn[offC + c0] + 1;
+  return g;
+}
+__device__ __forceinline__ int64_t xl_words(const XLGeom& g) {
+  return (int64_t)(g.nb - 1) * g.nrows + (int64_t)g.nsaved * g.nb * g.kds;
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup row scan: one table (or, with kXL, one column block of a table)
+// per workgroup of W waves, C columns per lane. Columns are counted from
+// jmax: t = jmax-1-j. Wave w owns the block t in [tb0 + w*64*C,
+// tb0 + (w+1)*64*C). Inside it, lanes own E contiguous columns per group and
+// the G = C/E groups are interleaved: lane l's element (g, e) is
+//   t = tb0 + w*64*C + g*64*E + l*E + e,   E = min(C, 4).
+// So one group of a TD row is a 64*E-int contiguous span that the wave
+// reads with one coalesced E-int load per lane, and the row scan is G
+// chained wave scans over groups.
+// ---------------------------------------------------------------------------
+template <int W, int C, bool kXL>
 __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   constexpr int E = C <= 4 ? C : 4;  // contiguous columns per group
   constexpr int G = C / E;           // groups per lane
   // TD ring slots (a power of two dividing 64): up to 32 registers of ring,
   // so narrow lanes prefetch more rows ahead.
-  constexpr int S = C >= 16 ? 2 : C >= 8 ? 4 : C >= 4 ? 8 : 16;
+  // (The column-blocked form also rings its imports: at most 8 slots.)
+  constexpr int S0 = C >= 16 ? 2 : C >= 8 ? 4 : C >= 4 ? 8 : 16;
+  constexpr int S = kXL && S0 > 8 ? 8 : S0;
   constexpr int PD = S - 1;  // rows of TD prefetched
+  constexpr int BW = 64 * W * C;                    // columns of one block
+  constexpr int KQ = kXL ? kMaxImports / 64 : 1;    // import entries per lane
   extern __shared__ uint16_t slots[];
   // Per-wave scan totals, double-buffered by row parity: a wave may write
   // row r-1's total while a slower wave still reads row r's.
   __shared__ int32_t xtot[2][W];
+  __shared__ int32_t xsh[kXL ? kMaxImports + 1 : 1];  // import flags, then the ticket
 
-  const int64_t tix = blockIdx.x;
+  int64_t tix = blockIdx.x;
+  int32_t blk = 0;
+  if constexpr (kXL) {
+    if (threadIdx.x == 0) xsh[kMaxImports] = (int32_t)atomicAdd(A.xl_ticket, 1u);
+    for (int k = threadIdx.x; k < kMaxImports; k += 64 * W) xsh[k] = 0;
+    __syncthreads();
+    const int32_t tk = xsh[kMaxImports];
+    tix = tk / A.xl_nbmax;
+    blk = tk % A.xl_nbmax;
+  }
   if (tix >= A.ntasks) return;
   const uint64_t task = A.tasks[tix];
   const PairMeta pm = A.pairs[task_pair(task)];
@@ -227,9 +250,12 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   const int32_t* __restrict__ orlR = A.pool.orl + offR;
   const int32_t* __restrict__ labR = A.pool.label + offR;
   const int32_t* __restrict__ kdepR = A.pool.kdep + offR;
+  const int32_t* __restrict__ lrankR = A.pool.lrank + offR;
   const int32_t* __restrict__ orlC = A.pool.orl + offC;
   const int32_t* __restrict__ labC = A.pool.label + offC;
   const int32_t* __restrict__ lrankC = A.pool.lrank + offC;
+  const int32_t* __restrict__ kdepC = A.pool.kdep + offC;
+  const int32_t* __restrict__ rtopC = A.pool.rtop + offC;
   const int32_t r0 = rows_x ? i0 : j0, c0 = rows_x ? j0 : i0;
   int32_t* TDo = A.td + (rows_x ? pm.td_off : pm.tdt_off);  // [r][c]
   int32_t* TDt = A.td + (rows_x ? pm.tdt_off : pm.td_off);  // [c][r]
@@ -238,32 +264,54 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
 
   const int32_t rmax = orlR[r0] + 1, cmax = orlC[c0] + 1;
   const int32_t ncols = cmax - c0;
+  const int32_t nrows = rmax - r0;
   const int32_t base = kdepR[r0];
   const int32_t orl_c0 = cmax - 1;
-  const int32_t lr_c0 = lrankC[c0];
-  // A slot row holds the compact far columns (leaves of subtree(c0) but the
-  // last) and, at index width, F[R][cmax] = rmax - R for the tree columns.
-  const int32_t width = lrankC[cmax] - lr_c0 - 1;
-  const int32_t rowlen = width + 1;
+  int32_t nb = 1;
+  if constexpr (kXL) {
+    nb = (ncols + BW - 1) / BW;
+    if (blk >= nb) return;
+  }
+  const int32_t tb0 = blk * BW;  // first t of this block (0 on chip)
+  // A slot row holds the compact far columns of this block (leaves but the
+  // table's last, by leaf rank), at index wb F[R][cmax] = rmax - R for the
+  // tree columns, and (kXL) the kds imported far values after it.
+  int32_t lbase, wb, kds = 0, kdep_c0 = 0;
+  if constexpr (kXL) {
+    const int32_t chi = cmax - 1 - tb0;
+    const int32_t clo = max(c0, cmax - tb0 - BW);
+    lbase = lrankC[max(clo - 1, c0)];
+    wb = max(0, lrankC[min(chi, cmax - 1)] - lbase);
+    kds = A.pool.kdn[offC + c0] + 1;
+    kdep_c0 = kdepC[c0];
+  } else {
+    lbase = lrankC[c0];
+    wb = lrankC[cmax] - lbase - 1;
+  }
+  const int32_t rowlen = wb + 1 + kds;
 
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int32_t tlane = wave * 64 * C + lane * E;  // t of this lane's element (0, 0)
+  const int32_t tlane = tb0 + wave * 64 * C + lane * E;  // t of this lane's element (0, 0)
 #define TCOL(g, e) (tlane + (g) * 64 * E + (e))
 
   // Per-column state, element j = g*E + e (t = TCOL(g, e), c = cmax-1-t).
   //   bfar: F[rmax][orl_C(c)+1] = cmax - orl_C(c) - 1 (the far value on the
   //         boundary row), or -1 on a tree column (orl_C(c) == orl_C(c0));
   //   labc: the column label (tree cells only);
-  //   foff: the column's far index in a slot row (width for tree columns);
+  //   foff: the column's far index in a slot row (wb for tree columns);
   //   woff: the index this column writes in a saved row (c-1 a leaf of
   //         subtree(c0)), or -1;
+  //   xinf: (kXL) for a written column whose right path's top keyroot lies
+  //         in a later block: that block << 16 | the path's depth index;
+  //         else -1;
   //   Fpm:  F[r+1][c] + 1 - t (the scan works on a[t] - t);
   //   negt: -t.
   // Columns past the table (t >= ncols) come after every real column in the
   // scan order, so whatever they hold never reaches a real column; only
   // their stores are masked.
   int32_t bfar[C], labc[C], foff[C], woff[C], Fpm[C], negt[C], tdq[S][C];
+  int32_t xinf[kXL ? C : 1];
   int32_t fpr0[G];  // F[r+1][c+1] of each group's element e = 0
 #pragma unroll
   for (int g = 0; g < G; ++g) {
@@ -281,11 +329,31 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       const bool tree = (oc == orl_c0);
       bfar[j] = tree ? -1 : cmax - oc - 1;
       labc[j] = lab;
-      foff[j] = tree ? width : lro - lr_c0;
-      woff[j] = (t < ncols && c > c0 && ocm == c - 1) ? lrm - lr_c0 : -1;
+      foff[j] = tree ? wb : lro - lbase;
+      const bool writes = t < ncols && c > c0 && ocm == c - 1;
+      woff[j] = writes ? lrm - lbase : -1;
       Fpm[j] = 2;  // F[rmax][c] + 1 - t = cmax - c + 1 - t
       negt[j] = -t;
+      if constexpr (kXL) {
+        // far column oc+1 in an earlier block: an import, by path depth
+        const bool imp = !tree && t < ncols && cmax - 2 - oc < tb0;
+        const int32_t kdc = kdepC[c] - kdep_c0;
+        if (imp) {
+          foff[j] = wb + 1 + kdc;
+          xsh[kdc] = 1;
+        }
+        const int32_t u = rtopC[max(c - 1, c0)];
+        const int32_t bu = (cmax - 1 - u) / BW;
+        xinf[j] = (writes && bu > blk) ? (bu << 16) | (kdepC[max(c - 1, c0)] - kdep_c0) : -1;
+      }
     }
+  }
+  // Which import entries this block uses (lane l holds depths l + 64q).
+  bool impl[KQ];
+  if constexpr (kXL) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < KQ; ++q) impl[q] = xsh[lane + 64 * q] != 0;
   }
 
   // TD row rr of this lane's columns into out[]. Loads are unconditional, so
@@ -318,26 +386,78 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   for (int q = 0; q < PD; ++q) load_row(max(rmax - 1 - q, r0), tdq[q]);
 
   // Row metadata, 64 rows per chunk: lane i holds row rmax-1-(64q+i) as
-  // (label, Rf = orl_R(r)+1, slots = fslot | wslot << 16 | wrow << 31), read
-  // per row with readlane; the next chunk is loaded one chunk ahead.
-  auto load_meta = [&](int q, int32_t& mlab, int32_t& mrf, int32_t& mslot) {
+  // (label, Rf = orl_R(r)+1, slots = fslot | wslot << 16 | wrow << 31) and,
+  // for kXL, the saved row's index (leaf rank of r-1) or -1; read per row
+  // with readlane; the next chunk is loaded one chunk ahead.
+  auto load_meta = [&](int q, int32_t& mlab, int32_t& mrf, int32_t& mslot, int32_t& mrid) {
     const int32_t rr = rmax - 1 - (64 * q + lane);
     const int32_t r = max(rr, r0);  // rows past the table read row r0
     const int32_t rp = max(r - 1, r0);
     const int32_t orr = orlR[r], lab = labR[r], kd = kdepR[r];
     const int32_t orp = orlR[rp], kdp = kdepR[rp];
     const int32_t fs = (orr + 1 < rmax) ? kd - base - 1 : 0;
-    const int32_t ws =
-        (r > r0 && orp == r - 1) ? (((kdp - base - 1) << 16) | (int32_t)0x80000000) : 0;
+    const bool saved = r > r0 && orp == r - 1;
+    const int32_t ws = saved ? (((kdp - base - 1) << 16) | (int32_t)0x80000000) : 0;
     mlab = lab;
     mrf = orr + 1;
     mslot = fs | ws;
+    if constexpr (kXL) mrid = (saved && rr >= r0) ? lrankR[rp] - lrankR[r0] : -1;
   };
-  const int32_t nrows = rmax - r0;
   const int32_t nchunks = (nrows + 63) >> 6;
-  int32_t mlab, mrf, mslot, nlab = 0, nrf = 0, nslot = 0;
-  load_meta(0, mlab, mrf, mslot);
-  if (nchunks > 1) load_meta(1, nlab, nrf, nslot);
+  int32_t mlab, mrf, mslot, mrid = -1, nlab = 0, nrf = 0, nslot = 0, nrid = -1;
+  load_meta(0, mlab, mrf, mslot, mrid);
+  if (nchunks > 1) load_meta(1, nlab, nrf, nslot, nrid);
+
+  // Column-blocked form: the upstream carry (cq) and this block's imports
+  // (iq, wave 0 only) of row k, validated, sit in ring slot k % S. They are
+  // read PD rows ahead and checked at the end of the row that read them, so
+  // a block runs at least PD rows behind the one before it.
+  unsigned long long* xw = A.xl_words;
+  int64_t cin_off = 0, cout_off = 0, exp_off = 0;
+  if constexpr (kXL) {
+    xw += A.xl_off[tix];
+    cin_off = (int64_t)(blk - 1) * nrows;
+    cout_off = (int64_t)blk * nrows;
+    exp_off = (int64_t)(nb - 1) * nrows;
+  }
+  int32_t cq[kXL ? S : 1], iq[kXL ? S : 1][KQ];
+  unsigned long long cw = 0, iw[KQ];
+  bool iwant[KQ];
+  int32_t irid = -1;
+  // Export word of saved row rid for block bb at depth index kx.
+  auto xword = [&](int32_t rid, int32_t bb, int32_t kx) {
+    return xw + exp_off + ((int64_t)rid * nb + bb) * kds + kx;
+  };
+  auto xl_issue = [&](int32_t k, int32_t rid) {  // row k of the table
+    if constexpr (kXL) {
+      if (blk > 0 && k < nrows) cw = xl_load(xw + cin_off + k);
+      irid = rid;
+#pragma unroll
+      for (int q = 0; q < KQ; ++q) {
+        iwant[q] = wave == 0 && impl[q] && rid >= 0 && k < nrows;
+        if (iwant[q]) iw[q] = xl_load(xword(rid, blk, lane + 64 * q));
+      }
+    }
+  };
+  auto xl_finish = [&](int32_t k, int u) {
+    if constexpr (kXL) {
+      if (blk > 0 && k < nrows) cq[u] = xl_wait(xw + cin_off + k, cw, true, A.xl_epoch, A.xl_abort);
+      if (wave == 0) {
+#pragma unroll
+        for (int q = 0; q < KQ; ++q)
+          iq[u][q] = xl_wait(xword(max(irid, 0), blk, lane + 64 * q), iw[q], iwant[q], A.xl_epoch,
+                             A.xl_abort);
+      }
+    }
+  };
+  if constexpr (kXL) {
+#pragma unroll
+    for (int k = 0; k < PD; ++k) {
+      if (k >= nrows) break;
+      xl_issue(k, __builtin_amdgcn_readlane(mrid, k));
+      xl_finish(k, k);
+    }
+  }
 
   for (int32_t q = 0; q < nchunks; ++q) {
     const int32_t rows_here = min(64, nrows - 64 * q);
@@ -352,6 +472,12 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
         if (i >= rows_here) break;
         const int32_t r = rmax - 1 - (64 * q + i);
         load_row(max(r - PD, r0), tdq[(u + PD) % S]);
+        if constexpr (kXL) {
+          const int32_t ia = i + PD;
+          const int32_t rid = ia < 64 ? __builtin_amdgcn_readlane(mrid, ia)
+                                      : __builtin_amdgcn_readlane(nrid, ia - 64);
+          xl_issue(64 * q + ia, rid);
+        }
 
         const int32_t labr = __builtin_amdgcn_readlane(mlab, i);
         const int32_t Rf = __builtin_amdgcn_readlane(mrf, i);
@@ -398,12 +524,24 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
         if (tree_row) row_terms(std::true_type{});
         else row_terms(std::false_type{});
 
-        int32_t carry = (rmax - r) + 1;  // F[r][cmax] - (-1)
+        // Carry into this wave: the boundary F[r][cmax] - (-1), or for a
+        // later column block the carry of the blocks before it, then the
+        // totals of this workgroup's earlier waves.
+        int32_t carry = (rmax - r) + 1;
+        if constexpr (kXL) {
+          if (blk > 0) carry = cq[u];
+        }
         if constexpr (W > 1) {
           int32_t* xt = xtot[r & 1];
           if (lane == 0) xt[wave] = carry_w;
           __syncthreads();
           for (int w = 0; w < wave; ++w) carry = min(carry, xt[w]);
+        }
+        if constexpr (kXL) {
+          // The next block's carry: this block's last wave, carry-in
+          // combined with its own total.
+          if (blk + 1 < nb && wave == W - 1 && lane == 0)
+            xl_store(xw + cout_off + (64 * q + i), A.xl_epoch, min(carry, carry_w));
         }
         // h becomes G - t; Fpm = G + 1 - t = h + 1.
 #pragma unroll
@@ -433,18 +571,36 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
 #pragma unroll
           for (int j = 0; j < C; ++j)
             if (woff[j] >= 0) wrow_ptr[woff[j]] = (uint16_t)(h[j] - negt[j]);
-          if (threadIdx.x == 0) wrow_ptr[width] = (uint16_t)(rmax - r);
+          if (threadIdx.x == 0) wrow_ptr[wb] = (uint16_t)(rmax - r);
+          if constexpr (kXL) {
+            // Imported far values of this saved row; far values later
+            // blocks import from this one.
+            if (wave == 0) {
+#pragma unroll
+              for (int q2 = 0; q2 < KQ; ++q2)
+                if (impl[q2]) wrow_ptr[wb + 1 + lane + 64 * q2] = (uint16_t)iq[u][q2];
+            }
+            const int32_t rid = __builtin_amdgcn_readlane(mrid, i);
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+              if (xinf[j] >= 0) {
+                const int32_t kx = xinf[j] & 0xFFFF;
+                for (int32_t bb = blk + 1; bb <= (xinf[j] >> 16); ++bb)
+                  xl_store(xword(rid, bb, kx), A.xl_epoch, h[j] - negt[j]);
+              }
+            }
+          }
           // The next row may read this slot row at columns of other waves.
           // (Every other LDS reuse is ordered by the barrier above: a slot
           // row is only rewritten after the rows that read it.)
           if constexpr (W > 1) __syncthreads();
         }
 
-        // F[r][c] at the column just before this wave's block (t = w*64*C-1).
-        // The previous wave's last element ends with h = min(its carry-in,
-        // its own total) = this wave's carry-in, and F = h + t; for wave 0
-        // this gives F[r][cmax] = rmax - r.
-        const int32_t left_in = carry + (wave * 64 * C - 1);
+        // F[r][c] at the column just before this wave's block (t = tb0 +
+        // w*64*C - 1). The previous wave's last element ends with h =
+        // min(its carry-in, its own total) = this wave's carry-in, and F =
+        // h + t; for wave 0 of block 0 this gives F[r][cmax] = rmax - r.
+        const int32_t left_in = carry + (tb0 + wave * 64 * C - 1);
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           const int jl = g * E + E - 1;  // the group's last column
@@ -452,24 +608,80 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
               g ? __builtin_amdgcn_readlane(h[jl - E] - negt[jl - E], 63) : left_in;
           fpr0[g] = wave_shr1(h[jl] - negt[jl], fill);
         }
+        if constexpr (kXL) xl_finish(64 * q + i + PD, (u + PD) % S);
       }
     }
     mlab = nlab;
     mrf = nrf;
     mslot = nslot;
-    if (q + 2 < nchunks) load_meta(q + 2, nlab, nrf, nslot);
+    mrid = nrid;
+    if (q + 2 < nchunks) load_meta(q + 2, nlab, nrf, nslot, nrid);
   }
 #undef TCOL
+}
+
+// Arena offsets of every column-blocked task of a batch, in bucket order
+// (level, then form), and their total (one workgroup).
+__global__ __launch_bounds__(1024) void xl_layout_kernel(XLLayoutArgs A) {
+  __shared__ unsigned long long part[1024];
+  __shared__ unsigned long long run;
+  const int tid = threadIdx.x;
+  if (tid == 0) run = 0;
+  __syncthreads();
+  int64_t ord = 0;
+  for (int L = 0; L < A.nlevels; ++L) {
+    for (int k = 0; k < kNumXLForms; ++k) {
+      const int bkt = L * kNumClasses + kFirstXLClass + k;
+      const int64_t cnt = (int64_t)A.counts[bkt];
+      const int64_t start = (int64_t)A.cursor[bkt] - cnt;  // cursors end past the bucket
+      for (int64_t c0 = 0; c0 < cnt; c0 += 1024) {
+        const int64_t idx = c0 + tid;
+        unsigned long long w = 0;
+        if (idx < cnt) {
+          const uint64_t task = A.tasks[start + idx];
+          w = (unsigned long long)xl_words(xl_geom(A.pool, A.pairs[task_pair(task)], task,
+                                                   xl_bw(k)));
+        }
+        part[tid] = w;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+          const unsigned long long v = tid >= off ? part[tid - off] : 0;
+          __syncthreads();
+          part[tid] += v;
+          __syncthreads();
+        }
+        if (idx < cnt) A.off[ord + idx] = (int64_t)(run + part[tid] - w);
+        __syncthreads();
+        if (tid == 1023) run += part[1023];
+        __syncthreads();
+      }
+      ord += cnt;
+    }
+  }
+  if (tid == 0) *A.total = run;
+}
+
+void launch_xl_layout(const XLLayoutArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(xl_layout_kernel, dim3(1), dim3(1024), 0, s, a);
+}
+
+// Zero fill for a newly grown column-blocked arena (epoch 0 is never used).
+__global__ void xl_zero_kernel(unsigned long long* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+void launch_xl_zero(unsigned long long* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(xl_zero_kernel, dim3(1024), dim3(256), 0, s, p, n);
 }
 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-typedef void (*RowKernel)(DPArgs);
-
 template <int W, int C>
 static void launch_row(const DPArgs& a, int lds_bytes, hipStream_t s) {
-  hipLaunchKernelGGL((rowscan_kernel<W, C>), dim3((unsigned)a.ntasks), dim3(64 * W),
+  hipLaunchKernelGGL((rowscan_kernel<W, C, false>), dim3((unsigned)a.ntasks), dim3(64 * W),
                      (size_t)lds_bytes, s, a);
 }
 
@@ -484,11 +696,17 @@ static void launch_row_c(int ci, const DPArgs& a, int lds, hipStream_t s) {
   }
 }
 
+template <int W, int C>
+static void launch_xl(const DPArgs& a, int lds_bytes, hipStream_t s) {
+  hipLaunchKernelGGL((rowscan_kernel<W, C, true>), dim3((unsigned)(a.ntasks * a.xl_nbmax)),
+                     dim3(64 * W), (size_t)lds_bytes, s, a);
+}
+
 // Grants every row-scan instantiation the LDS it may ask for (> 64 KiB needs
 // an explicit attribute).
-template <int W, int C>
+template <int W, int C, bool kXL = false>
 static hipError_t allow_lds(int limit) {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&rowscan_kernel<W, C>),
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&rowscan_kernel<W, C, kXL>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, limit);
 }
 template <int W>
@@ -506,7 +724,13 @@ hipError_t forest_init_attributes(int limit) {
   if ((e = allow_lds_c<2>(limit)) != hipSuccess) return e;
   if ((e = allow_lds_c<4>(limit)) != hipSuccess) return e;
   if ((e = allow_lds_c<8>(limit)) != hipSuccess) return e;
-  return allow_lds_c<16>(limit);
+  if ((e = allow_lds_c<16>(limit)) != hipSuccess) return e;
+  // column-blocked forms (xl_waves / xl_cpl)
+  if ((e = allow_lds<8, 8, true>(limit)) != hipSuccess) return e;
+  if ((e = allow_lds<8, 4, true>(limit)) != hipSuccess) return e;
+  if ((e = allow_lds<4, 4, true>(limit)) != hipSuccess) return e;
+  if ((e = allow_lds<2, 4, true>(limit)) != hipSuccess) return e;
+  return allow_lds<2, 2, true>(limit);
 }
 
 // Launches one (class, task list). Returns false for a class without a kernel.
@@ -523,7 +747,14 @@ bool launch_forest(int cls, const DPArgs& a, int lds_bytes, hipStream_t s) {
     }
     return true;
   }
-  return false;
+  switch (cls - kFirstXLClass) {  // must match xl_waves / xl_cpl
+    case 0: launch_xl<8, 8>(a, lds_bytes, s); return true;
+    case 1: launch_xl<8, 4>(a, lds_bytes, s); return true;
+    case 2: launch_xl<4, 4>(a, lds_bytes, s); return true;
+    case 3: launch_xl<2, 4>(a, lds_bytes, s); return true;
+    case 4: launch_xl<2, 2>(a, lds_bytes, s); return true;
+    default: return false;
+  }
 }
 
 }  // namespace xted

@@ -24,6 +24,7 @@ struct PrepArgs {
   int32_t* sleaf;
   int32_t* spre;
   int32_t* kdoff;
+  int32_t* rtop;
   TreeMeta* trees;
   int32_t* scratch;            // 6*(n+1) ints per tree at scratch_off[b]
   const int64_t* scratch_off;  // per block of this launch
@@ -47,6 +48,8 @@ struct PlanArgs {
   unsigned long long* cursor;  // scatter cursors (bucket offsets)
   int32_t* maxlds;             // nlevels*kNumClasses: largest slot storage per bucket
   uint64_t* tasks;
+  unsigned int* tickets;       // nlevels*kNumClasses: column-blocked launch counters
+  int32_t* abort_flag;         // column-blocked waits gave up
 };
 // Blocks a pair occupies in the plan kernels (same formula on both sides).
 inline int64_t plan_blocks(int32_t Kx, int32_t Ky, int32_t chunk) {
@@ -57,52 +60,26 @@ void launch_plan(const PlanArgs& a, int64_t nblocks, hipStream_t s);
 void launch_gather_dist(const PairMeta* pairs, const int32_t* td, int32_t npairs, int32_t* dist,
                         hipStream_t s);
 
-// Tables with a leaf keyroot, enumerated per level without a task list.
-struct LaneArgs {
+// Tables with a leaf keyroot, in closed form. For a single node v and a
+// subtree T (unit costs), TED({v}, T) = |T| - [label(v) occurs in T]: map v
+// onto a node of T with its label if there is one, insert the rest. So the
+// row of TD of a leaf keyroot i of x holds, for every node c of y,
+//   TD[i][c] = |sub_y(c)| - [label_x(i) occurs in sub_y(c)],
+// and a leaf keyroot j of y likewise gives the column TD[.][j]. In preorder,
+// sub(c) = [c, orl(c)], so the test is "the first position >= c holding the
+// label is <= orl(c)". These cells are exactly the tree cells of every table
+// with a leaf keyroot (the tables of X-TED_CPU/parallel_computing.cpp:4-45
+// with a one-node side), and they read nothing, so one launch covers them
+// all, independent of every level.
+struct LeafArgs {
   PoolView pool;
   const PairMeta* pairs;
+  const int64_t* line_start;  // npairs+1: prefix over pairs of K1_x + K1_y
   int32_t npairs;
-  int32_t level;                  // level of a lane launch
-  unsigned long long* offsets;    // nlevels*(npairs+1) prefix of rectangle sizes
-  unsigned long long* totals;     // nlevels: tables
-  unsigned long long* cells;      // nlevels: interior cells
   int32_t* td;
 };
-
-// |A| + |B| of pair p at level L (A: leaf keyroots of x by keyroots of y of
-// height L; B: non-leaf keyroots of x of height L by leaf keyroots of y).
-// When sizes is given it receives {|A|, |Y_L|, |X'_L|, |B|} and the start
-// positions {ylo, xlo} of Y_L and X'_L in the sorted lists.
-XTED_HD inline unsigned long long lane_rect_count(const LaneArgs& A, int p, int L, int32_t* sizes) {
-  const PairMeta pm = A.pairs[p];
-  const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
-  const int32_t nly = L <= ty.kdmax ? A.pool.kdoff[ty.node_off + L + 1] - A.pool.kdoff[ty.node_off + L] : 0;
-  const int32_t nlx = L <= tx.kdmax ? A.pool.kdoff[tx.node_off + L + 1] - A.pool.kdoff[tx.node_off + L] : 0;
-  const int32_t ny = (L == 0 ? ty.K1 : 0) + nly;
-  const int32_t ylo = L == 0 ? 0 : ty.K1 + (L <= ty.kdmax ? A.pool.kdoff[ty.node_off + L] : 0);
-  const int32_t xlo = tx.K1 + (L <= tx.kdmax ? A.pool.kdoff[tx.node_off + L] : 0);
-  const unsigned long long ca = (unsigned long long)tx.K1 * (unsigned long long)ny;
-  const unsigned long long cb = (unsigned long long)nlx * (unsigned long long)ty.K1;
-  if (sizes) {
-    sizes[0] = (int32_t)ca;
-    sizes[1] = ny;
-    sizes[2] = nlx;
-    sizes[3] = ylo;
-    sizes[4] = xlo;
-  }
-  return ca + cb;
-}
-// Interior cells of pair p's rectangles at level L (sizes from lane_rect_count).
-XTED_HD inline unsigned long long lane_rect_cells(const LaneArgs& A, int p, const int32_t* sz) {
-  const PairMeta pm = A.pairs[p];
-  const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
-  const int32_t* py = A.pool.spre + ty.node_off;
-  const int32_t* px = A.pool.spre + tx.node_off;
-  return (unsigned long long)tx.K1 * (unsigned long long)(py[sz[3] + sz[1]] - py[sz[3]]) +
-         (unsigned long long)ty.K1 * (unsigned long long)(px[sz[4] + sz[2]] - px[sz[4]]);
-}
-void launch_lane_offsets(const LaneArgs& a, int nlevels, hipStream_t s);
-void launch_lane_level(const LaneArgs& a, unsigned long long total, hipStream_t s);
+// One wave per line (a leaf keyroot of x or of y of one pair).
+void launch_leaf(const LeafArgs& a, int64_t nlines, hipStream_t s);
 
 // Ints of padding the TD arena keeps before and after every table, so the
 // row-scan kernels can issue whole-group loads without bounds branches.
@@ -115,12 +92,33 @@ struct DPArgs {
   int32_t* td;            // arena holding every TD / TDT
   const uint64_t* tasks;  // this launch's tables
   int64_t ntasks;
+  // Column-blocked launches only.
+  unsigned long long* xl_words;  // cross-workgroup words of every such task
+  const int64_t* xl_off;         // this launch's tasks' offsets into xl_words
+  unsigned int* xl_ticket;       // (task, block) counter of this launch, zeroed
+  int32_t* xl_abort;             // raised when a wait gives up
+  uint32_t xl_epoch;             // tag of this launch's words (never 0)
+  int32_t xl_nbmax;              // workgroups per task in the grid
 };
+// Arena layout of the column-blocked tasks of a batch (after the scatter).
+struct XLLayoutArgs {
+  PoolView pool;
+  const PairMeta* pairs;
+  const uint64_t* tasks;
+  const unsigned long long* counts;
+  const unsigned long long* cursor;  // scatter cursors: bucket ends
+  int32_t nlevels;
+  int64_t* off;                 // per column-blocked task, in bucket order
+  unsigned long long* total;    // words
+};
+void launch_xl_layout(const XLLayoutArgs& a, hipStream_t s);
+void launch_xl_zero(unsigned long long* p, int64_t n, hipStream_t s);
 // Lets every row-scan kernel request up to lds_limit bytes of dynamic LDS.
 hipError_t forest_init_attributes(int lds_limit);
 // Launches the row-scan kernel of class `cls` over a.tasks[0..ntasks).
-// Returns false when the class has no kernel (the lane class runs per level
-// through launch_lane_level, the XL class has none).
+// Returns false when the class has no kernel (the leaf class runs as one
+// launch_leaf, class kClassXL has none). Column-blocked classes launch
+// a.ntasks * a.xl_nbmax workgroups.
 bool launch_forest(int cls, const DPArgs& a, int lds_bytes, hipStream_t s);
 
 }  // namespace xted

@@ -41,18 +41,30 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
   // the row keyroot (at least one: the kernels read slot 0 unconditionally),
   // each as long as the column keyroot's leaf count.
   const int32_t slots = max(a <= b ? kdx : kdy, 1);
-  const int64_t need = (int64_t)slots * (a <= b ? A.pool.sleaf[ly] : A.pool.sleaf[kx]) * 2;
-  int cls = table_class(a, b, need, A.rule);
-  if (cls == kClassLane && !A.lane_enable) {
-    // single-row tables routed to the narrowest row-scan class that fits
-    const int cols = a <= b ? b : a;
-    int c = (cols + 63) / 64, ci = ilog2_ceil(c);
-    cls = ci > 4 ? kClassXL : row_class(0, ci);
+  const int32_t leaves_c = a <= b ? A.pool.sleaf[ly] : A.pool.sleaf[kx];
+  int64_t need = (int64_t)slots * leaves_c * 2;
+  // With the leaf kernel off, single-row tables take the row-scan forms.
+  int cls = table_class(a, b, need, a <= b ? kdx : kdy, a <= b ? kdy : kdx, leaves_c,
+                        A.lane_enable != 0, A.rule, &need);
+  // A forced column-blocked form takes every multi-row table it can hold.
+  if (A.force_class >= kFirstXLClass && cls >= 1 && cls != kClassXL) {
+    const int k = A.force_class - kFirstXLClass;
+    const int32_t kd_c = a <= b ? kdy : kdx;
+    const int64_t n = xl_lds_need(a <= b ? kdx : kdy, kd_c, leaves_c, xl_bw(k));
+    if (kd_c + 1 <= kMaxImports && n <= A.rule.lds_max) {
+      cls = A.force_class;
+      need = n;
+    }
   }
-  if (A.force_class >= 1 && cls >= 1 && cls <= kNumRowClasses) {
+  if (A.force_class >= 1 && A.force_class <= kNumRowClasses && cls >= 1 &&
+      cls <= kNumRowClasses) {
     const int wi = (A.force_class - 1) / 5, ci = (A.force_class - 1) % 5;
     const int cols = a <= b ? b : a;
-    if (cols <= 64 * (1 << wi) * (1 << ci) && need <= A.rule.lds_max) cls = A.force_class;
+    const int64_t row_need = (int64_t)slots * leaves_c * 2;
+    if (cls < kClassXL && cols <= 64 * (1 << wi) * (1 << ci) && row_need <= A.rule.lds_max) {
+      cls = A.force_class;
+      need = row_need;
+    }
   }
   *bucket = level * kNumClasses + cls;
   *lds = (int)need;
@@ -192,74 +204,27 @@ __global__ __launch_bounds__(1024) void plan_scan_kernel(const unsigned long lon
   }
 }
 
-// Clears the count / max arrays (a kernel rather than a memset so the plan
-// can be captured into a graph).
-__global__ void plan_clear_kernel(unsigned long long* counts, unsigned long long* cells, int nb,
-                                  int32_t* maxlds) {
+// Clears the count / max / ticket arrays and the abort flag (a kernel rather
+// than a memset so the plan can be captured into a graph).
+__global__ void plan_clear_kernel(PlanArgs a, int nb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nb) {
-    counts[i] = 0;
-    cells[i] = 0;
-    maxlds[i] = 0;
+    a.counts[i] = 0;
+    a.cells[i] = 0;
+    a.maxlds[i] = 0;
+    a.tickets[i] = 0;
   }
+  if (i == 0) *a.abort_flag = 0;
 }
 
 void launch_plan(const PlanArgs& a, int64_t nblocks, hipStream_t s) {
   const int nb = a.nlevels * kNumClasses;
-  hipLaunchKernelGGL(plan_clear_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, a.counts,
-                     a.cells, nb, a.maxlds);
+  hipLaunchKernelGGL(plan_clear_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, a, nb);
   if (nblocks > 0)
     hipLaunchKernelGGL(plan_kernel<false>, dim3((unsigned)nblocks), dim3(kPlanThreads), 0, s, a);
   hipLaunchKernelGGL(plan_scan_kernel, dim3(1), dim3(1024), 0, s, a.counts, a.cursor, nb);
   if (nblocks > 0)
     hipLaunchKernelGGL(plan_kernel<true>, dim3((unsigned)nblocks), dim3(kPlanThreads), 0, s, a);
-}
-
-// Tables with a leaf keyroot, per level L and pair p, form two rectangles
-// of the sorted keyroot lists: A = (leaf keyroots of x) x (keyroots of y at
-// height L) and B = (non-leaf keyroots of x at height L) x (leaf keyroots of
-// y). Their level is the other keyroot's height. One block per level writes
-// the exclusive prefix over pairs of |A| + |B| (npairs+1 entries) and the
-// level total.
-__global__ __launch_bounds__(1024) void lane_offsets_kernel(LaneArgs A) {
-  __shared__ unsigned long long part[1024];
-  const int L = blockIdx.x, tid = threadIdx.x;
-  const int np = A.npairs;
-  const int per = (np + 1023) / 1024;
-  const int lo = min(np, tid * per), hi = min(np, lo + per);
-  unsigned long long* out = A.offsets + (int64_t)L * (np + 1);
-  unsigned long long s = 0, cells = 0;
-  for (int p = lo; p < hi; ++p) {
-    int32_t sz[5];
-    s += lane_rect_count(A, p, L, sz);
-    cells += lane_rect_cells(A, p, sz);
-  }
-  part[tid] = s;
-  __shared__ unsigned long long csum;
-  if (tid == 0) csum = 0;
-  __syncthreads();
-  atomicAdd(&csum, cells);
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    unsigned long long v = tid >= off ? part[tid - off] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
-  }
-  unsigned long long run = tid ? part[tid - 1] : 0;
-  for (int p = lo; p < hi; ++p) {
-    out[p] = run;
-    run += lane_rect_count(A, p, L, nullptr);
-  }
-  if (tid == 1023) {
-    out[np] = part[1023];
-    A.totals[L] = part[1023];
-    A.cells[L] = csum;
-  }
-}
-
-void launch_lane_offsets(const LaneArgs& a, int nlevels, hipStream_t s) {
-  hipLaunchKernelGGL(lane_offsets_kernel, dim3(nlevels), dim3(1024), 0, s, a);
 }
 
 // Distances TD[0][0] of every pair into dist[].

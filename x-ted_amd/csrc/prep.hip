@@ -145,7 +145,10 @@ __global__ __launch_bounds__(kPrepThreads) void tree_prep_kernel(PrepArgs A) {
     if (!changed) break;
     __syncthreads();
   }
-  for (int i = tid; i < n; i += kPrepThreads) kdep[i] = cc[ptr[i]];
+  for (int i = tid; i < n; i += kPrepThreads) {
+    kdep[i] = cc[ptr[i]];
+    A.rtop[off + i] = ptr[i];
+  }
   __syncthreads();
 
   // ---- keyroot nesting height ------------------------------------------------

@@ -1,0 +1,106 @@
+"""Summarises rocprofv3 output of bench.py runs into the short text kept
+under profiles/.
+
+  kernels:  python x-ted_amd/rocprof_summary.py kernels <kernel_trace.csv> [--bench bench.json]
+            per kernel: dispatches, average and total duration; with the
+            bench JSON line, the algorithmic bytes per launch of each class
+            (4 bytes of TD per interior cell) and the rate they imply.
+  traffic:  python x-ted_amd/rocprof_summary.py traffic KERNEL <fetch.csv> <write.csv>
+                <copy_fetch.csv> <copy_write.csv> --copy-bytes N
+            FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the dispatches of
+            KERNEL, each scaled by what the counter reads per KiB of a plain
+            copy of N bytes, and their sum as HBM bytes per launch.
+"""
+import argparse
+import csv
+import json
+import sys
+from collections import defaultdict
+
+
+def short(name):
+    """'rowscan_w4_c4', 'leaf_kernel', ... from a demangled kernel name."""
+    base = name.split("(")[0].replace("void ", "").replace("xted::", "")
+    if base.startswith("rowscan_kernel<"):
+        w, c = base[len("rowscan_kernel<"):-1].split(",")
+        return f"rowscan_w{int(w)}_c{int(c)}"
+    return base
+
+
+def kernels(path, bench=None):
+    per = defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            per[short(r["Kernel_Name"])].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    classes = {}
+    if bench:
+        with open(bench) as f:
+            for line in f:
+                if line.startswith("{"):
+                    classes = json.loads(line).get("classes", {})
+    rows = sorted(per.items(), key=lambda kv: -sum(kv[1]))
+    out = [f"{'kernel':<26}{'dispatches':>11}{'avg_us':>11}{'total_ms':>11}"
+           f"{'alg_MB/launch':>15}{'alg_GB/s':>10}"]
+    for k, d in rows:
+        avg = sum(d) / len(d) / 1e3
+        line = f"{k:<26}{len(d):>11}{avg:>11.1f}{sum(d) / 1e6:>11.2f}"
+        c = classes.get(k) or (classes.get("leaf") if k == "leaf_kernel" else None)
+        if c and k.startswith("rowscan"):
+            b = 4 * c["cells"] / c["launches"]
+            line += f"{b / 1e6:>15.1f}{b / (avg * 1e3):>10.1f}"
+        out.append(line)
+    return "\n".join(out)
+
+
+def _per_dispatch(path, kernel):
+    vals = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if kernel is None or short(r["Kernel_Name"]) == kernel:
+                vals.append(float(r["Counter_Value"]))
+    return vals
+
+
+def traffic(kernel, fetch, write, cfetch, cwrite, copy_bytes):
+    f = _per_dispatch(fetch, kernel)
+    w = _per_dispatch(write, kernel)
+    # The copy run's largest dispatch is the copy itself.
+    cf = max(_per_dispatch(cfetch, None))
+    cw = max(_per_dispatch(cwrite, None))
+    kf = cf * 1024 / copy_bytes  # counter KiB per KiB read
+    kw = cw * 1024 / copy_bytes
+    fk, wk = sum(f) / len(f), sum(w) / len(w)
+    hbm = (fk / kf + wk / kw) * 1024
+    return "\n".join([
+        f"kernel: {kernel}",
+        f"dispatches: {len(f)}",
+        f"fetch_kib_per_dispatch: {fk:.1f}",
+        f"write_kib_per_dispatch: {wk:.1f}",
+        f"copy_calibration: {copy_bytes} bytes copied; FETCH_SIZE {cf:.1f} KiB "
+        f"(factor {kf:.4f}), WRITE_SIZE {cw:.1f} KiB (factor {kw:.4f})",
+        f"hbm_bytes_per_launch: {hbm:.0f}",
+    ])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    k = sub.add_parser("kernels")
+    k.add_argument("trace")
+    k.add_argument("--bench")
+    t = sub.add_parser("traffic")
+    t.add_argument("kernel")
+    t.add_argument("fetch")
+    t.add_argument("write")
+    t.add_argument("copy_fetch")
+    t.add_argument("copy_write")
+    t.add_argument("--copy-bytes", type=int, required=True)
+    a = ap.parse_args()
+    if a.cmd == "kernels":
+        print(kernels(a.trace, a.bench))
+    else:
+        print(traffic(a.kernel, a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

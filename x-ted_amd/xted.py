@@ -26,17 +26,22 @@ ERR_CANCELLED = 10
 
 CLASS_AUTO = -1
 CLASS_LANE = 0
-NUM_CLASSES = 27
+NUM_CLASSES = 32
+XL_BW = (4096, 2048, 1024, 512, 256)  # column-block widths of classes 27..31
 
 
 def class_name(cls):
-    """'lane' or 'rowscan_w{W}_c{C}' (W waves, C columns per lane) or 'xl'."""
+    """'leaf' (closed form for tables with a leaf keyroot) or
+    'rowscan_w{W}_c{C}' (W waves, C columns per lane) or 'xl_bw{B}' (column
+    blocks of B columns, one workgroup each) or 'too_large'."""
     if cls == 0:
-        return "lane"
+        return "leaf"
     if 1 <= cls <= 25:
         wi, ci = divmod(cls - 1, 5)
         return f"rowscan_w{1 << wi}_c{1 << ci}"
-    return "xl"
+    if 27 <= cls < 32:
+        return f"xl_bw{XL_BW[cls - 27]}"
+    return "too_large"
 
 GEN_RRT, GEN_DEEP, GEN_SHALLOW = 0, 1, 2
 
