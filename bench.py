@@ -15,7 +15,12 @@ The same JSON line also reports:
   * the dominant kernel's roofline: 4 bytes of TD per cell against 8 TB/s
     HBM, timed with events around each of its launches;
   * the reference's own X-TED_CPU (compiled from its sources into
-    oracle/_ref/) on a bounded sample of the same pairs, on this host.
+    oracle/_ref/) on a bounded sample of the same pairs, on this host;
+  * the extra-large pair (two 20,000-node random recursive trees): GCUPS
+    and algorithmic HBM rate;
+  * serving: a pool of 10,000 trees (50..5000 nodes, log-uniform), Poisson
+    arrivals of random pairs at a fixed rate through the continuous-batching
+    server, p50/p99 latency and completed pairs/s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -37,6 +42,7 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 4
 CONFIG_BATCH = 3
 CONFIG_SERVE = 5
+CONFIG_XL = 4
 
 
 def batch_trees(npairs, cfg=CONFIG_BATCH):
@@ -115,6 +121,34 @@ def serve_bench(trees, seconds, rate, budget, max_batch, seed=7):
     }
 
 
+def xl_pair_bench(reps=3):
+    """The extra-large pair (two 20,000-node random recursive trees, alphabet
+    500): its widest tables take the column-blocked form. Distance only
+    (the 1.6 GB table stays in HBM); median of `reps` runs."""
+    x = xted.generate(xted.GEN_RRT, 20000, 500, xted.derive_seed(CONFIG_XL, 0))
+    y = xted.generate(xted.GEN_RRT, 20000, 500, xted.derive_seed(CONFIG_XL, 1))
+    with xted.Context(0) as ctx:
+        ids = ctx.pool_add([x, y])
+        xs, ys = ids[:1].copy(), ids[1:].copy()
+        ctx.batch(xs, ys)  # warm
+        ctx.set_profile(True)
+        runs = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            d, _, st = ctx.batch(xs, ys)
+            runs.append((time.perf_counter() - t, st, int(d[0])))
+        prof = ctx.class_profile()
+    wall, st, dist = sorted(runs, key=lambda r: r[0])[len(runs) // 2]
+    xl = {xted.class_name(c): round(v[3], 2) for c, v in prof.items() if c >= 27}
+    return {"pair": "rrt20000_a500", "dist": dist, "ms": round(wall * 1e3, 2),
+            "gcups": round(st.cells / wall / 1e9, 2),
+            "hbm_gbs_algorithmic": round(BYTES_PER_CELL * st.cells / wall / 1e9, 1),
+            "frac_of_hbm_peak": round(BYTES_PER_CELL * st.cells / wall / 1e9 / HBM_PEAK_GBS, 4),
+            "cells": st.cells, "tables": st.tables, "levels": st.levels,
+            "plan_ms": round(st.plan_ms, 2), "dp_ms": round(st.dp_ms, 2),
+            "column_blocked_ms": xl}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -182,7 +216,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the granularity rule first and use the fastest")
-    ap.add_argument("--serve-seconds", type=float, default=0.0,
+    ap.add_argument("--no-xl", action="store_true", help="skip the 20,000-node pair")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="also run the serving leg (Poisson arrivals) for this long")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-trees", type=int, default=10000)
@@ -261,6 +296,11 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(trees, args.cpu_budget)
 
+    xl = None
+    if not args.no_xl:
+        xl = xl_pair_bench()
+        log(f"20k pair: {xl}")
+
     serving = None
     if args.serve_seconds > 0:
         t = time.perf_counter()
@@ -296,6 +336,7 @@ def main():
         "roofline": roofline,
         "classes": classes,
         "cpu_baseline": cpu,
+        "xl_pair": xl,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

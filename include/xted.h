@@ -114,6 +114,11 @@ int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t max_cpl,
  * a column-blocked one (27..31); XTED_CLASS_AUTO restores the rule.
  * lane_enable = 0 sends single-row tables to the row-scan kernels. */
 int xted_force_class(xted_context* ctx, int32_t cls, int32_t lane_enable);
+/* Levels of a batch that at most tail_pairs pairs reach (the last levels of
+ * the deepest pairs) hold few tables, so their time is one table's latency:
+ * there the rule takes the fewest waves, up to 16 columns per lane and slot
+ * storage up to the device limit. 0 turns this off. */
+int xted_set_tail_rule(xted_context* ctx, int32_t tail_pairs);
 /* Streams (1 to 4) that the launches of one level run on concurrently. The
  * reference gives each class kernel its own stream but waits for each
  * launch before the next (X-TED_GPU/parallel_X_TED.cu:433-438). Results do

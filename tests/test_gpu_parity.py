@@ -279,3 +279,18 @@ def test_large_pair_transposed(gpu_ctx):
     d2, t2, _ = gpu_ctx.ted(y, x)
     assert d1 == d2 == want["dist"]
     assert np.array_equal(t1, t2.T)
+
+
+@pytest.mark.parametrize("tail_pairs", [0, 3, 1 << 20])
+def test_tail_rule_does_not_change_results(gpu_ctx, tail_pairs):
+    """The latency rule for sparse top levels (off, some levels, every level)
+    re-classes tables; every table must stay bit-exact."""
+    trees, xs, ys, shapes = _mixed_batch(gpu_ctx, 16, 4)
+    try:
+        gpu_ctx.set_tail_rule(tail_pairs)
+        dist, tds, _ = gpu_ctx.batch(xs, ys, td_shapes=shapes)
+    finally:
+        gpu_ctx.set_tail_rule(32)
+    for k in range(16):
+        od, otd, _ = oracle.ted(trees[2 * k], trees[2 * k + 1], threads=8)
+        assert dist[k] == od and np.array_equal(tds[k], otd)

@@ -24,18 +24,20 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 import xted  # noqa: E402
 
-DEFAULT = {"lds_per_wave": 10 * 1024, "max_cpl": 8, "min_waves": 1, "wide_cols": 1 << 30}
-GRID = {
-    "lds_per_wave": [5 * 1024, 10 * 1024, 20 * 1024, 40 * 1024],
-    "max_cpl": [2, 4, 8],
-    "min_waves": [1],
-    "wide_cols": [1 << 30],
-}
+DEFAULT = {"lds_per_wave": 10 * 1024, "max_cpl": 8, "min_waves": 1, "wide_cols": 1 << 30,
+           "tail_pairs": 32}
+# Searched in stages: the tail rule first, then the throughput rule with the
+# best tail setting.
+STAGES = [
+    {"tail_pairs": [0, 8, 32, 128, 512]},
+    {"lds_per_wave": [5 * 1024, 10 * 1024, 20 * 1024], "max_cpl": [4, 8]},
+]
 TUNING_FILE = os.path.join(HERE, "tuning.json")
 
 
 def apply(ctx, rule):
     ctx.set_tuning(rule["lds_per_wave"], rule["max_cpl"], rule["min_waves"], rule["wide_cols"])
+    ctx.set_tail_rule(rule["tail_pairs"])
 
 
 def load(path=TUNING_FILE):
@@ -43,7 +45,7 @@ def load(path=TUNING_FILE):
     try:
         with open(path) as f:
             rule = json.load(f)["rule"]
-        return {k: int(rule[k]) for k in DEFAULT}
+        return {k: int(rule.get(k, DEFAULT[k])) for k in DEFAULT}
     except (OSError, KeyError, ValueError):
         return dict(DEFAULT)
 
@@ -61,29 +63,31 @@ def time_rule(ctx, xs, ys, rule, reps=2):
     return best, dist
 
 
-def tune(ctx, xs, ys, grid=GRID, log=print):
-    """Returns (best_rule, results) over the grid; the default rule's
+def tune(ctx, xs, ys, stages=STAGES, log=print):
+    """Returns (best_rule, results) over the staged grid; the default rule's
     distances are the reference every candidate must match."""
     base_t, base_d = time_rule(ctx, xs, ys, DEFAULT)
     results = [(dict(DEFAULT), base_t)]
     log(f"default {DEFAULT}: {base_t * 1e3:.1f} ms")
     best, best_t = dict(DEFAULT), base_t
-    keys = list(grid)
-    for vals in itertools.product(*(grid[k] for k in keys)):
-        rule = dict(zip(keys, vals))
-        if rule == DEFAULT:
-            continue
-        try:
-            t, d = time_rule(ctx, xs, ys, rule)
-        except xted.XtedError as e:
-            log(f"{rule}: rejected ({e})")
-            continue
-        if not np.array_equal(d, base_d):
-            raise RuntimeError(f"rule {rule} changed a distance")
-        results.append((rule, t))
-        log(f"{rule}: {t * 1e3:.1f} ms")
-        if t < best_t:
-            best, best_t = rule, t
+    for grid in stages:
+        keys = list(grid)
+        start = dict(best)
+        for vals in itertools.product(*(grid[k] for k in keys)):
+            rule = dict(start, **dict(zip(keys, vals)))
+            if any(rule == r for r, _ in results):
+                continue
+            try:
+                t, d = time_rule(ctx, xs, ys, rule)
+            except xted.XtedError as e:
+                log(f"{rule}: rejected ({e})")
+                continue
+            if not np.array_equal(d, base_d):
+                raise RuntimeError(f"rule {rule} changed a distance")
+            results.append((rule, t))
+            log(f"{rule}: {t * 1e3:.1f} ms")
+            if t < best_t:
+                best, best_t = rule, t
     apply(ctx, best)
     return best, results
 

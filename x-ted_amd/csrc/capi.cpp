@@ -213,6 +213,15 @@ int xted_force_class(xted_context* ctx, int32_t cls, int32_t lane_enable) {
   return OK;
 }
 
+int xted_set_tail_rule(xted_context* ctx, int32_t tail_pairs) {
+  if (!ctx || tail_pairs < 0) {
+    set_error("xted_set_tail_rule: invalid argument");
+    return ERR_INVALID_ARG;
+  }
+  ctx->ctx.tail_pairs = tail_pairs;
+  return OK;
+}
+
 int xted_set_level_streams(xted_context* ctx, int32_t n) {
   if (!ctx || n < 1 || n > kMaxLevelStreams) {
     set_error("xted_set_level_streams: n must be 1 to " + std::to_string(kMaxLevelStreams));

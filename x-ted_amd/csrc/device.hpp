@@ -104,6 +104,9 @@ constexpr int kLdsPerWave = 10 * 1024;
 constexpr int kLdsMaxTable = 150 * 1024;
 constexpr int kMaxRowWaves = 16;
 constexpr int kMaxCols = 64 * 16 * kMaxRowWaves;  // 16384
+// Widest table the on-chip forms take (W = 16 waves x 8 columns per lane;
+// 16 columns per lane at 16 waves would spill); wider tables are column-blocked.
+constexpr int kMaxOnChipCols = 64 * 8 * kMaxRowWaves;  // 8192
 
 // Tuning knobs carried by the plan builder (autotuner writes these).
 struct ClassRule {
@@ -139,7 +142,8 @@ XTED_HD inline int table_class(int a, int b, int64_t lds_need, int kd_r, int kd_
   const int cols = a <= b ? b : a;
   *need = lds_need;
   if (rows == 1 && leaf_form) return kClassLane;
-  if (cols > 64 * rule.max_cpl * kMaxRowWaves || lds_need > rule.lds_max) {
+  const int cpl_cap = rule.max_cpl < 8 ? rule.max_cpl : 8;
+  if (cols > 64 * cpl_cap * kMaxRowWaves || lds_need > rule.lds_max) {
     if (kd_c + 1 > kMaxImports) return kClassXL;
     for (int k = 0; k < kNumXLForms; ++k) {
       const int64_t n = xl_lds_need(kd_r, kd_c, leaves_c, xl_bw(k));

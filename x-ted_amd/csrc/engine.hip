@@ -80,7 +80,7 @@ Context::~Context() {
                     &krleaf_, &trees_, &prep_scratch_, &prep_off_, &pairs_, &blk_, &counts_,
                     &cells_, &cursor_, &maxlds_, &tasks_, &td_, &dist_, &skr_, &skd_, &ssize_,
                     &sleaf_, &spre_, &kdoff_, &rtop_, &lines_, &tickets_, &abort_, &xl_off_,
-                    &xl_total_, &xl_words_})
+                    &xl_total_, &xl_words_, &tail_lv_})
     b->release();
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
@@ -329,7 +329,16 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     st.cells += info_[x].ksum * info_[y].ksum;
   }
   const int nb = nlevels * kNumClasses;
+  // Levels that at most tail_pairs pairs reach use the latency rule.
+  std::vector<int32_t> active(nlevels + 1, 0);
+  for (int32_t i = 0; i < npairs; ++i) ++active[meta_[xs[i]].kdmax + meta_[ys[i]].kdmax];
+  h_tail_.assign(nlevels, 0);
+  for (int L = nlevels - 1, run = 0; L >= 0; --L) {
+    run += active[L];
+    h_tail_[L] = (tail_pairs > 0 && run <= tail_pairs) ? 1 : 0;
+  }
   ST_TRY(pairs_.grow(sizeof(PairMeta) * npairs, false, stream_));
+  ST_TRY(tail_lv_.grow(nlevels, false, stream_));
   ST_TRY(blk_.grow(sizeof(int64_t) * (npairs + 1), false, stream_));
   ST_TRY(counts_.grow(sizeof(unsigned long long) * nb, false, stream_));
   ST_TRY(cells_.grow(sizeof(unsigned long long) * nb, false, stream_));
@@ -347,6 +356,7 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
                          hipMemcpyHostToDevice, stream_));
   HIP_TRY(hipMemcpyAsync(lines_.p, lines.data(), sizeof(int64_t) * (npairs + 1),
                          hipMemcpyHostToDevice, stream_));
+  HIP_TRY(hipMemcpyAsync(tail_lv_.p, h_tail_.data(), nlevels, hipMemcpyHostToDevice, stream_));
   HIP_TRY(hipEventRecord(ev_[0], stream_));
 
   PlanArgs pa;
@@ -357,6 +367,8 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   pa.nlevels = nlevels;
   pa.lchunk_tables = plan_chunk;
   pa.rule = rule;
+  pa.rule_tail = ClassRule{lds_limit, 1, 1 << 30, lds_limit, 16};
+  pa.tail_level = tail_lv_.as<uint8_t>();
   pa.force_class = force_class;
   pa.lane_enable = lane_enable;
   pa.counts = counts_.as<unsigned long long>();

@@ -87,6 +87,10 @@ class Context {
   int32_t plan_chunk = 2048;  // tables per plan block
   int32_t profile = 0;        // time every forest launch with events
   int32_t level_streams = kMaxLevelStreams;  // streams one level's launches spread over
+  // Levels reached by at most this many pairs of a batch take the latency
+  // rule (fewest waves, up to 16 columns per lane, LDS up to the device
+  // limit); 0 turns it off.
+  int32_t tail_pairs = 32;
   ClassProfile class_profile[kNumClasses];
 
   hipStream_t stream() const { return stream_; }
@@ -111,7 +115,8 @@ class Context {
   DevBuf lines_;
   // column-blocked tables: launch tickets, abort flag, per-task arena offsets,
   // the arena of cross-workgroup words and the tag of the latest launch
-  DevBuf tickets_, abort_, xl_off_, xl_total_, xl_words_;
+  DevBuf tickets_, abort_, xl_off_, xl_total_, xl_words_, tail_lv_;
+  std::vector<uint8_t> h_tail_;
   uint32_t xl_epoch_ = 0;
   std::vector<unsigned long long> h_counts_, h_cells_;
   std::vector<int32_t> h_maxlds_;  // largest slot storage per (level, class) bucket

@@ -41,6 +41,10 @@ struct PlanArgs {
   int32_t nlevels;
   int32_t lchunk_tables;       // target tables per block
   ClassRule rule;
+  // Levels reached by only a few pairs hold few tables, so their time is the
+  // latency of one table: tail_level[L] != 0 selects rule_tail there.
+  ClassRule rule_tail;
+  const uint8_t* tail_level;   // nlevels
   int32_t force_class;         // -1 = automatic; else row-scan tables go to this class if they fit
   int32_t lane_enable;         // 0 = single-row tables use the row-scan kernels
   unsigned long long* counts;  // nlevels*kNumClasses

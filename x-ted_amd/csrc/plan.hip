@@ -37,6 +37,7 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
   *cells = (unsigned long long)a * (unsigned long long)b;
   const int32_t kdx = A.pool.skd[kx], kdy = A.pool.skd[ly];
   const int32_t level = kdx + kdy;
+  const ClassRule& rule = A.tail_level[level] ? A.rule_tail : A.rule;
   // Slot storage of the oriented table: one uint16 row per nesting level of
   // the row keyroot (at least one: the kernels read slot 0 unconditionally),
   // each as long as the column keyroot's leaf count.
@@ -45,7 +46,7 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
   int64_t need = (int64_t)slots * leaves_c * 2;
   // With the leaf kernel off, single-row tables take the row-scan forms.
   int cls = table_class(a, b, need, a <= b ? kdx : kdy, a <= b ? kdy : kdx, leaves_c,
-                        A.lane_enable != 0, A.rule, &need);
+                        A.lane_enable != 0, rule, &need);
   // A forced column-blocked form takes every multi-row table it can hold.
   if (A.force_class >= kFirstXLClass && cls >= 1 && cls != kClassXL) {
     const int k = A.force_class - kFirstXLClass;
