@@ -224,9 +224,21 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--xl-only", action="store_true",
+                    help="only the 20,000-node pair (for profiling it alone)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the serving leg (for profiling it alone)")
     args = ap.parse_args()
     if args.calibrate_copy:
         copy_calibration(args.calibrate_copy)
+        return
+    if args.xl_only:
+        print(json.dumps({"xl_pair": xl_pair_bench()}), flush=True)
+        return
+    if args.serve_only:
+        strees = serving_trees(args.serve_trees)
+        print(json.dumps({"serving": serve_bench(strees, args.serve_seconds, args.serve_rate,
+                                                 budget=32 << 30, max_batch=4096)}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

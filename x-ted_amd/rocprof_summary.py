@@ -5,6 +5,10 @@ under profiles/.
             per kernel: dispatches, average and total duration; with the
             bench JSON line, the algorithmic bytes per launch of each class
             (4 bytes of TD per interior cell) and the rate they imply.
+  counters: python x-ted_amd/rocprof_summary.py counters <sq.csv> <lds.csv>
+            per kernel from the SQ counter runs: waves, VALU / LDS / VMEM
+            instructions per wave, VALU-active and waiting shares of wave
+            cycles, LDS bank-conflict share of LDS-active cycles.
   traffic:  python x-ted_amd/rocprof_summary.py traffic KERNEL <fetch.csv> <write.csv>
                 <copy_fetch.csv> <copy_write.csv> --copy-bytes N
             FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the dispatches of
@@ -22,8 +26,11 @@ def short(name):
     """'rowscan_w4_c4', 'leaf_kernel', ... from a demangled kernel name."""
     base = name.split("(")[0].replace("void ", "").replace("xted::", "")
     if base.startswith("rowscan_kernel<"):
-        w, c = base[len("rowscan_kernel<"):-1].split(",")
-        return f"rowscan_w{int(w)}_c{int(c)}"
+        parts = [p.strip() for p in base[len("rowscan_kernel<"):-1].split(",")]
+        w, c = int(parts[0]), int(parts[1])
+        if len(parts) > 2 and parts[2] == "true":
+            return f"xl_bw{64 * w * c}"
+        return f"rowscan_w{w}_c{c}"
     return base
 
 
@@ -44,11 +51,34 @@ def kernels(path, bench=None):
     for k, d in rows:
         avg = sum(d) / len(d) / 1e3
         line = f"{k:<26}{len(d):>11}{avg:>11.1f}{sum(d) / 1e6:>11.2f}"
-        c = classes.get(k) or (classes.get("leaf") if k == "leaf_kernel" else None)
+        c = classes.get(k)
         if c and k.startswith("rowscan"):
             b = 4 * c["cells"] / c["launches"]
             line += f"{b / 1e6:>15.1f}{b / (avg * 1e3):>10.1f}"
         out.append(line)
+    return "\n".join(out)
+
+
+def _sums(path):
+    per = defaultdict(lambda: defaultdict(float))
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            per[short(r["Kernel_Name"])][r["Counter_Name"]] += float(r["Counter_Value"])
+    return per
+
+
+def counters(sq_path, lds_path):
+    sq, lds = _sums(sq_path), _sums(lds_path)
+    out = [f"{'kernel':<22}{'waves':>12}{'valu/wave':>11}{'lds/wave':>10}{'vmem/wave':>10}"
+           f"{'valu_active':>12}{'waiting':>9}{'lds_conflict':>13}"]
+    for k in sorted(sq, key=lambda k: -sq[k]["SQ_BUSY_CYCLES"]):
+        a, b = sq[k], lds.get(k, {})
+        w = max(a["SQ_WAVES"], 1)
+        cyc = max(a["SQ_WAVE_CYCLES"], 1)
+        out.append(f"{k:<22}{a['SQ_WAVES']:>12.0f}{a['SQ_INSTS_VALU'] / w:>11.0f}"
+                   f"{a['SQ_INSTS_LDS'] / w:>10.0f}{a['SQ_INSTS_VMEM_RD'] / w:>10.0f}"
+                   f"{a['SQ_ACTIVE_INST_VALU'] / cyc:>12.3f}{a['SQ_WAIT_INST_ANY'] / cyc:>9.3f}"
+                   f"{b.get('SQ_LDS_BANK_CONFLICT', 0) / max(b.get('SQ_LDS_IDX_ACTIVE', 0), 1):>13.3f}")
     return "\n".join(out)
 
 
@@ -88,6 +118,9 @@ def main():
     k = sub.add_parser("kernels")
     k.add_argument("trace")
     k.add_argument("--bench")
+    c = sub.add_parser("counters")
+    c.add_argument("sq")
+    c.add_argument("lds")
     t = sub.add_parser("traffic")
     t.add_argument("kernel")
     t.add_argument("fetch")
@@ -98,6 +131,8 @@ def main():
     a = ap.parse_args()
     if a.cmd == "kernels":
         print(kernels(a.trace, a.bench))
+    elif a.cmd == "counters":
+        print(counters(a.sq, a.lds))
     else:
         print(traffic(a.kernel, a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes))
 
