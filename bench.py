@@ -112,6 +112,8 @@ def serve_bench(trees, seconds, rate, budget, max_batch, seed=7):
     return {
         "offered_rate": rate, "seconds": seconds, "queries": n_arr,
         "completed": len(ok), "rejected": sum(1 for v in res.values() if v[0] != 0),
+        "errors": {str(c): sum(1 for v in res.values() if v[0] == c)
+                   for c in sorted({v[0] for v in res.values() if v[0] != 0})},
         "pairs_per_s": round(len(ok) / span, 1),
         "p50_ms": round(float(np.percentile(lat, 50)), 2) if len(lat) else None,
         "p99_ms": round(float(np.percentile(lat, 99)), 2) if len(lat) else None,

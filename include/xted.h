@@ -38,8 +38,8 @@ extern "C" {
 /* Granularity classes: 0 = tables with a leaf keyroot (closed form, one
  * launch per batch); 1 + wi*5 + ci = workgroup row scan with 2^wi waves and
  * 2^ci columns per lane (wi, ci in 0..4; the classes xted_force_class
- * takes); 26 = no form fits (XTED_ERR_TOO_LARGE); 27..31 = column blocks
- * of 4096, 2048, 1024, 512 or 256 columns, one workgroup per block, for
+ * takes); 26 = no form fits (XTED_ERR_TOO_LARGE); 27..32 = column blocks
+ * of 4096, 2048, 1024, 512, 256 or 128 columns, one workgroup per block, for
  * tables too wide or too LDS-hungry for one workgroup (the role of the
  * reference's multi-block and last-table kernels,
  * X-TED_GPU/parallel_X_TED.cu:883-1131). */
@@ -111,7 +111,7 @@ int xted_batch(xted_context* ctx, int32_t npairs, const int32_t* x_ids, const in
 int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t max_cpl,
                     int32_t min_waves, int32_t wide_cols);
 /* Forces every table that fits into class cls, a row-scan class (1..25) or
- * a column-blocked one (27..31); XTED_CLASS_AUTO restores the rule.
+ * a column-blocked one (27..32); XTED_CLASS_AUTO restores the rule.
  * lane_enable = 0 sends single-row tables to the row-scan kernels. */
 int xted_force_class(xted_context* ctx, int32_t cls, int32_t lane_enable);
 /* Levels of a batch that at most tail_pairs pairs reach (the last levels of

@@ -216,11 +216,11 @@ def test_granularity_rules_do_not_change_results(gpu_ctx, rule):
         assert got[0][k] == od and np.array_equal(got[1][k], otd)
 
 
-@pytest.mark.parametrize("cls", [27, 28, 29, 30, 31])
+@pytest.mark.parametrize("cls", [27, 28, 29, 30, 31, 32])
 def test_every_column_blocked_form_forced(gpu_ctx, cls):
     """Every multi-row table goes through one column-blocked form (blocks of
-    4096 .. 256 columns, one workgroup each, chained per row through device
-    memory). With 256-column blocks a 1200-column table spans 5 blocks, so
+    4096 .. 128 columns, one workgroup each, chained per row through device
+    memory). With 128-column blocks a 1200-column table spans 10 blocks, so
     carries and imported far values cross several block boundaries."""
     gpu_ctx.force_class(cls)
     try:

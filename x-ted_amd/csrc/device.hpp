@@ -80,19 +80,19 @@ XTED_HD inline uint32_t task_j0(uint64_t t) { return (uint32_t)t & 0xFFFFFu; }
 // Classes 1..25: workgroup row scan with W = 2^wi waves and C = 2^ci columns
 //   per lane, class = 1 + wi*5 + ci (wi, ci in 0..4).
 // Class 26: tables no form below can hold (an error).
-// Classes 27..31: column-blocked form for tables too wide or too LDS-hungry
+// Classes 27..32: column-blocked form for tables too wide or too LDS-hungry
 //   for one workgroup: blocks of xl_bw(k) columns, one workgroup each,
 //   chained per row through device memory, class = 27 + k.
 constexpr int kClassLane = 0;  // the leaf class
 constexpr int kNumRowClasses = 25;
 constexpr int kClassXL = 26;
 constexpr int kFirstXLClass = 27;
-constexpr int kNumXLForms = 5;
-constexpr int kNumClasses = kFirstXLClass + kNumXLForms;  // 32
+constexpr int kNumXLForms = 6;
+constexpr int kNumClasses = kFirstXLClass + kNumXLForms;  // 33
 // Column-blocked forms: waves and columns per lane of one block.
 XTED_HD inline int xl_waves(int k) { return k <= 1 ? 8 : k == 2 ? 4 : 2; }
-XTED_HD inline int xl_cpl(int k) { return k == 0 ? 8 : k <= 3 ? 4 : 2; }
-XTED_HD inline int xl_bw(int k) { return 64 * xl_waves(k) * xl_cpl(k); }  // 4096 .. 256
+XTED_HD inline int xl_cpl(int k) { return k == 0 ? 8 : k <= 3 ? 4 : k == 4 ? 2 : 1; }
+XTED_HD inline int xl_bw(int k) { return 64 * xl_waves(k) * xl_cpl(k); }  // 4096 .. 128
 // Far values a block takes from earlier blocks are indexed by the nesting
 // depth of their right path below the column keyroot: at most this many.
 constexpr int kMaxImports = 256;

@@ -46,7 +46,8 @@ int DevBuf::grow(size_t bytes, bool keep, hipStream_t s) {
   size_t ncap = std::max(bytes, cap + cap / 2);
   ncap = (ncap + 4095) & ~(size_t)4095;
   void* np = nullptr;
-  hipError_t e = hipMalloc(&np, ncap);
+  hipError_t e = fine ? hipExtMallocWithFlags(&np, ncap, hipDeviceMallocFinegrained)
+                      : hipMalloc(&np, ncap);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     set_error("device allocation of " + std::to_string(ncap) + " bytes failed: " +
@@ -74,22 +75,23 @@ Context::Context(int device) : device_(device) {}
 Context::~Context() {
   if (stream_) {
     (void)hipSetDevice(device_);
-    (void)hipStreamSynchronize(stream_);
+    (void)hipDeviceSynchronize();
   }
+  batch_.release();
   for (DevBuf* b : {&parent_, &label_, &orl_, &kdep_, &kdn_, &lrank_, &kr_, &krkd_, &krsize_,
-                    &krleaf_, &trees_, &prep_scratch_, &prep_off_, &pairs_, &blk_, &counts_,
-                    &cells_, &cursor_, &maxlds_, &tasks_, &td_, &dist_, &skr_, &skd_, &ssize_,
-                    &sleaf_, &spre_, &kdoff_, &rtop_, &lines_, &tickets_, &abort_, &xl_off_,
-                    &xl_total_, &xl_words_, &tail_lv_})
+                    &krleaf_, &trees_, &prep_scratch_, &prep_off_, &skr_, &skd_, &ssize_, &sleaf_,
+                    &spre_, &kdoff_, &rtop_})
     b->release();
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : prof_ev_) (void)hipEventDestroy(e);
   if (fork_ev_) (void)hipEventDestroy(fork_ev_);
+  if (plan_done_) (void)hipEventDestroy(plan_done_);
   for (auto& e : join_ev_)
     if (e) (void)hipEventDestroy(e);
   for (auto& sd : side_)
     if (sd) (void)hipStreamDestroy(sd);
+  if (plan_stream_) (void)hipStreamDestroy(plan_stream_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -107,9 +109,11 @@ int Context::init() {
   }
   HIP_TRY(hipSetDevice(device_));
   HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&plan_stream_, hipStreamNonBlocking));
   for (auto& sd : side_) HIP_TRY(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
   for (auto& ev : ev_) HIP_TRY(hipEventCreate(&ev));
   HIP_TRY(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&plan_done_, hipEventDisableTiming));
   for (auto& ev : join_ev_) HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   // Slot storage limit: what the device grants one workgroup, less the
   // row-scan kernels' static LDS, capped at kLdsMaxTable.
@@ -281,28 +285,60 @@ int Context::tree_arrays(int32_t id, int32_t* orl, int32_t* kr, int32_t* K, int3
   return OK;
 }
 
-int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int32_t* dist,
-                       int32_t* const* td_out, BatchStats* stats) {
-  std::lock_guard<std::mutex> lk(mu_);
-  const auto t_start = std::chrono::steady_clock::now();
-  BatchStats st;
-  if (npairs < 0 || (npairs > 0 && (!xs || !ys || !dist))) {
-    set_error("run_batch: null argument");
-    return ERR_INVALID_ARG;
+int xl_timeout(const int32_t* rec);
+
+void Group::release() {
+  if (h_dist_pin) (void)hipHostFree(h_dist_pin);
+  h_dist_pin = nullptr;
+  pin_cap = 0;
+  if (done_ev) (void)hipEventDestroy(done_ev);
+  done_ev = nullptr;
+  for (DevBuf* b : {&pairs, &blk, &counts, &cells, &cursor, &maxlds, &tickets, &abort, &tasks, &td,
+                    &dist, &lines_d, &tail_lv, &xl_off, &xl_total, &xl_words})
+    b->release();
+}
+
+int Context::lane_streams(int lane, hipStream_t* out) const {
+  if (lane >= 0 && lane < kMaxLevelStreams) {
+    out[0] = lane == 0 ? stream_ : side_[lane - 1];
+    return 1;
   }
-  if (npairs == 0) {
-    if (stats) *stats = st;
-    return OK;
+  out[0] = stream_;
+  for (int k = 1; k < kMaxLevelStreams; ++k) out[k] = side_[k - 1];
+  return std::max(1, std::min(level_streams, kMaxLevelStreams));
+}
+
+hipStream_t Context::lane_stream(int lane) const {
+  hipStream_t s[kMaxLevelStreams];
+  lane_streams(lane, s);
+  return s[0];
+}
+
+int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys, int lane) {
+  BatchStats& st = g.st;
+  st = BatchStats{};
+  g.npairs = 0;
+  if (npairs <= 0 || !xs || !ys) {
+    set_error("plan: null argument");
+    return ERR_INVALID_ARG;
   }
   if (npairs >= (1 << kTaskPairBits)) {
     set_error("run_batch: too many pairs in one batch");
     return ERR_TOO_LARGE;
   }
   HIP_TRY(hipSetDevice(device_));
-  std::vector<PairMeta> pm(npairs);
-  std::vector<int64_t> blk(npairs + 1, 0), lines(npairs + 1, 0);
-  int64_t td_total = 0, planned = 0, leaf_tables = 0, leaf_cells = 0;
-  int32_t nlevels = 0, maxn = 1;
+  // The plan runs on its own stream, so a server can plan a new group while
+  // the levels of others run; it returns once the plan is on the host.
+  hipStream_t ps = plan_stream_;
+  g.xs.assign(xs, xs + npairs);
+  g.ys.assign(ys, ys + npairs);
+  g.pm.resize(npairs);
+  std::vector<int64_t> blk(npairs + 1, 0);
+  g.lines.assign(npairs + 1, 0);
+  int64_t td_total = 0, planned = 0;
+  g.leaf_tables = g.leaf_cells = 0;
+  int32_t nlevels = 0;
+  g.maxn = 1;
   for (int32_t i = 0; i < npairs; ++i) {
     const int32_t x = xs[i], y = ys[i];
     if (x < 0 || y < 0 || x >= (int32_t)meta_.size() || y >= (int32_t)meta_.size()) {
@@ -311,16 +347,16 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     }
     const TreeMeta &a = meta_[x], &b = meta_[y];
     const int64_t mn = (int64_t)a.n * b.n;
-    maxn = std::max(maxn, std::max(a.n, b.n));
-    pm[i] = PairMeta{x, y, td_total + kTdPad, td_total + 2 * kTdPad + mn};
+    g.maxn = std::max(g.maxn, std::max(a.n, b.n));
+    g.pm[i] = PairMeta{x, y, td_total + kTdPad, td_total + 2 * kTdPad + mn};
     td_total += 2 * mn + 3 * kTdPad;
     // With the leaf kernel on, the plan covers only non-leaf keyroot pairs;
     // the leaf kernel writes one line per leaf keyroot of either tree.
     const int32_t ka = lane_enable ? a.K - a.K1 : a.K, kb = lane_enable ? b.K - b.K1 : b.K;
-    lines[i + 1] = lines[i] + (lane_enable ? a.K1 + b.K1 : 0);
+    g.lines[i + 1] = g.lines[i] + (lane_enable ? a.K1 + b.K1 : 0);
     if (lane_enable) {
-      leaf_tables += (int64_t)a.K1 * b.K + (int64_t)(a.K - a.K1) * b.K1;
-      leaf_cells += (int64_t)a.K1 * info_[y].ksum + (int64_t)b.K1 * (info_[x].ksum - a.K1);
+      g.leaf_tables += (int64_t)a.K1 * b.K + (int64_t)(a.K - a.K1) * b.K1;
+      g.leaf_cells += (int64_t)a.K1 * info_[y].ksum + (int64_t)b.K1 * (info_[x].ksum - a.K1);
     }
     blk[i + 1] = blk[i] + plan_blocks(ka, kb, plan_chunk);
     planned += (int64_t)ka * kb;
@@ -328,82 +364,77 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     st.tables += (int64_t)a.K * b.K;
     st.cells += info_[x].ksum * info_[y].ksum;
   }
+  g.td_bytes = (int64_t)sizeof(int32_t) * td_total;
   const int nb = nlevels * kNumClasses;
   // Levels that at most tail_pairs pairs reach use the latency rule.
   std::vector<int32_t> active(nlevels + 1, 0);
   for (int32_t i = 0; i < npairs; ++i) ++active[meta_[xs[i]].kdmax + meta_[ys[i]].kdmax];
-  h_tail_.assign(nlevels, 0);
+  g.h_tail.assign(nlevels, 0);
   for (int L = nlevels - 1, run = 0; L >= 0; --L) {
     run += active[L];
-    h_tail_[L] = (tail_pairs > 0 && run <= tail_pairs) ? 1 : 0;
+    g.h_tail[L] = (tail_pairs > 0 && run <= tail_pairs) ? 1 : 0;
   }
-  ST_TRY(pairs_.grow(sizeof(PairMeta) * npairs, false, stream_));
-  ST_TRY(tail_lv_.grow(nlevels, false, stream_));
-  ST_TRY(blk_.grow(sizeof(int64_t) * (npairs + 1), false, stream_));
-  ST_TRY(counts_.grow(sizeof(unsigned long long) * nb, false, stream_));
-  ST_TRY(cells_.grow(sizeof(unsigned long long) * nb, false, stream_));
-  ST_TRY(cursor_.grow(sizeof(unsigned long long) * nb, false, stream_));
-  ST_TRY(maxlds_.grow(sizeof(int32_t) * nb, false, stream_));
-  ST_TRY(tickets_.grow(sizeof(unsigned int) * nb, false, stream_));
-  ST_TRY(abort_.grow(sizeof(int32_t), false, stream_));
-  ST_TRY(tasks_.grow(sizeof(uint64_t) * std::max<int64_t>(planned, 1), false, stream_));
-  ST_TRY(td_.grow(sizeof(int32_t) * td_total, false, stream_));
-  ST_TRY(dist_.grow(sizeof(int32_t) * npairs, false, stream_));
-  ST_TRY(lines_.grow(sizeof(int64_t) * (npairs + 1), false, stream_));
-  HIP_TRY(hipMemcpyAsync(pairs_.p, pm.data(), sizeof(PairMeta) * npairs, hipMemcpyHostToDevice,
-                         stream_));
-  HIP_TRY(hipMemcpyAsync(blk_.p, blk.data(), sizeof(int64_t) * (npairs + 1),
-                         hipMemcpyHostToDevice, stream_));
-  HIP_TRY(hipMemcpyAsync(lines_.p, lines.data(), sizeof(int64_t) * (npairs + 1),
-                         hipMemcpyHostToDevice, stream_));
-  HIP_TRY(hipMemcpyAsync(tail_lv_.p, h_tail_.data(), nlevels, hipMemcpyHostToDevice, stream_));
-  HIP_TRY(hipEventRecord(ev_[0], stream_));
+  ST_TRY(g.pairs.grow(sizeof(PairMeta) * npairs, false, ps));
+  ST_TRY(g.tail_lv.grow(nlevels, false, ps));
+  ST_TRY(g.blk.grow(sizeof(int64_t) * (npairs + 1), false, ps));
+  ST_TRY(g.counts.grow(sizeof(unsigned long long) * nb, false, ps));
+  ST_TRY(g.cells.grow(sizeof(unsigned long long) * nb, false, ps));
+  ST_TRY(g.cursor.grow(sizeof(unsigned long long) * nb, false, ps));
+  ST_TRY(g.maxlds.grow(sizeof(int32_t) * nb, false, ps));
+  ST_TRY(g.tickets.grow(sizeof(unsigned int) * nb, false, ps));
+  ST_TRY(g.abort.grow(sizeof(int32_t) * 8, false, ps));
+  ST_TRY(g.tasks.grow(sizeof(uint64_t) * std::max<int64_t>(planned, 1), false, ps));
+  ST_TRY(g.td.grow((size_t)g.td_bytes, false, ps));
+  ST_TRY(g.dist.grow(sizeof(int32_t) * npairs, false, ps));
+  ST_TRY(g.lines_d.grow(sizeof(int64_t) * (npairs + 1), false, ps));
+  HIP_TRY(hipMemcpyAsync(g.pairs.p, g.pm.data(), sizeof(PairMeta) * npairs,
+                         hipMemcpyHostToDevice, ps));
+  HIP_TRY(hipMemcpyAsync(g.blk.p, blk.data(), sizeof(int64_t) * (npairs + 1),
+                         hipMemcpyHostToDevice, ps));
+  HIP_TRY(hipMemcpyAsync(g.lines_d.p, g.lines.data(), sizeof(int64_t) * (npairs + 1),
+                         hipMemcpyHostToDevice, ps));
+  HIP_TRY(hipMemcpyAsync(g.tail_lv.p, g.h_tail.data(), nlevels, hipMemcpyHostToDevice, ps));
+  HIP_TRY(hipEventRecord(ev_[0], ps));
 
   PlanArgs pa;
   pa.pool = view();
-  pa.pairs = pairs_.as<PairMeta>();
-  pa.blk_start = blk_.as<int64_t>();
+  pa.pairs = g.pairs.as<PairMeta>();
+  pa.blk_start = g.blk.as<int64_t>();
   pa.npairs = npairs;
   pa.nlevels = nlevels;
   pa.lchunk_tables = plan_chunk;
   pa.rule = rule;
   pa.rule_tail = ClassRule{lds_limit, 1, 1 << 30, lds_limit, 16};
-  pa.tail_level = tail_lv_.as<uint8_t>();
+  pa.tail_level = g.tail_lv.as<uint8_t>();
   pa.force_class = force_class;
   pa.lane_enable = lane_enable;
-  pa.counts = counts_.as<unsigned long long>();
-  pa.cells = cells_.as<unsigned long long>();
-  pa.cursor = cursor_.as<unsigned long long>();
-  pa.maxlds = maxlds_.as<int32_t>();
-  pa.tasks = tasks_.as<uint64_t>();
-  pa.tickets = tickets_.as<unsigned int>();
-  pa.abort_flag = abort_.as<int32_t>();
-  launch_plan(pa, blk[npairs], stream_);
+  pa.counts = g.counts.as<unsigned long long>();
+  pa.cells = g.cells.as<unsigned long long>();
+  pa.cursor = g.cursor.as<unsigned long long>();
+  pa.maxlds = g.maxlds.as<int32_t>();
+  pa.tasks = g.tasks.as<uint64_t>();
+  pa.tickets = g.tickets.as<unsigned int>();
+  pa.abort_flag = g.abort.as<int32_t>();
+  launch_plan(pa, blk[npairs], ps);
   HIP_TRY(hipGetLastError());
-  LeafArgs la;
-  la.pool = pa.pool;
-  la.pairs = pa.pairs;
-  la.line_start = lines_.as<int64_t>();
-  la.npairs = npairs;
-  la.td = td_.as<int32_t>();
-  h_counts_.assign(nb, 0);
-  h_maxlds_.assign(nb, 0);
-  HIP_TRY(hipMemcpyAsync(h_counts_.data(), counts_.p, sizeof(unsigned long long) * nb,
-                         hipMemcpyDeviceToHost, stream_));
-  HIP_TRY(hipMemcpyAsync(h_maxlds_.data(), maxlds_.p, sizeof(int32_t) * nb,
-                         hipMemcpyDeviceToHost, stream_));
-  h_cells_.assign(nb, 0);
-  HIP_TRY(hipMemcpyAsync(h_cells_.data(), cells_.p, sizeof(unsigned long long) * nb,
-                         hipMemcpyDeviceToHost, stream_));
-  HIP_TRY(hipStreamSynchronize(stream_));
-  HIP_TRY(hipEventRecord(ev_[1], stream_));
+  g.h_counts.assign(nb, 0);
+  g.h_maxlds.assign(nb, 0);
+  g.h_cells.assign(nb, 0);
+  HIP_TRY(hipMemcpyAsync(g.h_counts.data(), g.counts.p, sizeof(unsigned long long) * nb,
+                         hipMemcpyDeviceToHost, ps));
+  HIP_TRY(hipMemcpyAsync(g.h_maxlds.data(), g.maxlds.p, sizeof(int32_t) * nb,
+                         hipMemcpyDeviceToHost, ps));
+  HIP_TRY(hipMemcpyAsync(g.h_cells.data(), g.cells.p, sizeof(unsigned long long) * nb,
+                         hipMemcpyDeviceToHost, ps));
+  HIP_TRY(hipStreamSynchronize(ps));
+  HIP_TRY(hipEventRecord(ev_[1], ps));
   // Column-blocked tables: ordinal of each bucket's first task, then (with a
-  // second short sync, only in batches that have such tables) the arena
+  // second short sync, only in plans that have such tables) the arena
   // layout computed on the device.
-  std::vector<int64_t> xl_first(nb, 0);
-  int64_t nxl = 0;
+  g.xl_first.assign(nb, 0);
+  g.nxl = 0;
   for (int L = 0; L < nlevels; ++L) {
-    if (h_counts_[(size_t)L * kNumClasses + kClassXL]) {
+    if (g.h_counts[(size_t)L * kNumClasses + kClassXL]) {
       set_error("run_batch: a table fits no form (nesting height above " +
                 std::to_string(kMaxImports - 1) + " or slot storage above " +
                 std::to_string(lds_limit) + " bytes even in 256-column blocks)");
@@ -411,13 +442,13 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     }
     for (int k = 0; k < kNumXLForms; ++k) {
       const size_t bkt = (size_t)L * kNumClasses + kFirstXLClass + k;
-      xl_first[bkt] = nxl;
-      nxl += (int64_t)h_counts_[bkt];
+      g.xl_first[bkt] = g.nxl;
+      g.nxl += (int64_t)g.h_counts[bkt];
     }
   }
-  if (nxl > 0) {
-    ST_TRY(xl_off_.grow(sizeof(int64_t) * nxl, false, stream_));
-    ST_TRY(xl_total_.grow(sizeof(unsigned long long), false, stream_));
+  if (g.nxl > 0) {
+    ST_TRY(g.xl_off.grow(sizeof(int64_t) * g.nxl, false, ps));
+    ST_TRY(g.xl_total.grow(sizeof(unsigned long long), false, ps));
     XLLayoutArgs xa;
     xa.pool = pa.pool;
     xa.pairs = pa.pairs;
@@ -425,119 +456,225 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     xa.counts = pa.counts;
     xa.cursor = pa.cursor;
     xa.nlevels = nlevels;
-    xa.off = xl_off_.as<int64_t>();
-    xa.total = xl_total_.as<unsigned long long>();
-    launch_xl_layout(xa, stream_);
+    xa.off = g.xl_off.as<int64_t>();
+    xa.total = g.xl_total.as<unsigned long long>();
+    launch_xl_layout(xa, ps);
     HIP_TRY(hipGetLastError());
     unsigned long long words = 0;
-    HIP_TRY(hipMemcpyAsync(&words, xa.total, sizeof(words), hipMemcpyDeviceToHost, stream_));
-    HIP_TRY(hipStreamSynchronize(stream_));
+    HIP_TRY(hipMemcpyAsync(&words, xa.total, sizeof(words), hipMemcpyDeviceToHost, ps));
+    HIP_TRY(hipStreamSynchronize(ps));
     const size_t bytes = sizeof(unsigned long long) * std::max<unsigned long long>(words, 1);
-    if (bytes > xl_words_.cap) {
-      ST_TRY(xl_words_.grow(bytes, false, stream_));
-      launch_xl_zero(xl_words_.as<unsigned long long>(), (int64_t)(xl_words_.cap / 8), stream_);
+    if (bytes > g.xl_words.cap) {
+      ST_TRY(g.xl_words.grow(bytes, false, ps));
+      launch_xl_zero(g.xl_words.as<unsigned long long>(), (int64_t)(g.xl_words.cap / 8), ps);
       HIP_TRY(hipGetLastError());
     }
   }
-  DPArgs da;
-  da.pool = pa.pool;
-  da.pairs = pa.pairs;
-  da.td = td_.as<int32_t>();
-  da.xl_words = xl_words_.as<unsigned long long>();
-  da.xl_abort = abort_.as<int32_t>();
-  // With profiling on, an event pair brackets every launch (same stream).
-  std::vector<std::pair<int, int>> prof;  // (class, bucket or -1-level) per timed launch
-  if (profile) {
-    size_t need = 0;
-    for (unsigned long long c : h_counts_) need += c ? 2 : 0;
-    need += 2;
-    while (prof_ev_.size() < need) {
+  // Level launches (main stream, and the side streams forked from it) come
+  // after everything the plan enqueued, the arena fill included.
+  HIP_TRY(hipEventRecord(plan_done_, ps));
+  HIP_TRY(hipStreamWaitEvent(lane_stream(lane), plan_done_, 0));
+  g.npairs = npairs;
+  g.nlevels = nlevels;
+  g.next_level = 0;
+  g.level_off = 0;
+  st.levels = nlevels;
+  return OK;
+}
+
+int Context::launch_levels(Group* const* gs, int ng, std::vector<std::pair<int, int>>* prof,
+                           int lane) {
+  // Launches of one level are independent, and so are levels of different
+  // groups: they fork from the main stream onto up to level_streams streams
+  // (largest first) and join before the next tick, so latency-bound launches
+  // of few large tables overlap with the rest instead of queueing behind it.
+  struct Launch {
+    Group* g;
+    int cls, bucket;
+    unsigned long long off, cnt, cells;
+  };
+  std::vector<Launch> lv;
+  for (int gi = 0; gi < ng; ++gi) {
+    Group& g = *gs[gi];
+    const int L = g.next_level;
+    if (L >= g.nlevels) continue;
+    // The leaf lines read nothing and level-0 tables (two chains) read no TD,
+    // so the leaf kernel runs beside level 0.
+    if (L == 0 && g.lines[g.npairs] > 0)
+      lv.push_back({&g, kClassLane, -1, 0, (unsigned long long)g.lines[g.npairs],
+                    (unsigned long long)g.leaf_cells});
+    unsigned long long off = g.level_off;
+    for (int c = 0; c < kNumClasses; ++c) {
+      const size_t b = (size_t)L * kNumClasses + c;
+      if (g.h_counts[b]) lv.push_back({&g, c, (int)b, off, g.h_counts[b], g.h_cells[b]});
+      off += g.h_counts[b];
+    }
+    g.level_off = off;
+    ++g.next_level;
+  }
+  std::stable_sort(lv.begin(), lv.end(),
+                   [](const Launch& x, const Launch& y) { return x.cells > y.cells; });
+  if (prof) {
+    while (prof_ev_.size() < 2 * (prof->size() + lv.size())) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
       prof_ev_.push_back(e);
     }
   }
-  // Launches of one level are independent: they fork from the main stream
-  // onto up to level_streams streams (largest first) and join before the
-  // next level, so latency-bound launches of few large tables overlap with
-  // the rest of their level instead of queueing behind it.
-  struct Launch {
-    int cls, bucket;
-    unsigned long long off, cnt, cells;
-  };
-  std::vector<Launch> lv;
-  unsigned long long off = 0;
-  for (int L = 0; L < nlevels; ++L) {
-    lv.clear();
-    // The leaf lines read nothing and level-0 tables (two chains) read no TD,
-    // so the leaf kernel runs beside level 0.
-    if (L == 0 && lines[npairs] > 0)
-      lv.push_back({kClassLane, -1, 0, (unsigned long long)lines[npairs],
-                    (unsigned long long)leaf_cells});
-    for (int c = 0; c < kNumClasses; ++c) {
-      const size_t b = (size_t)L * kNumClasses + c;
-      if (h_counts_[b]) lv.push_back({c, (int)b, off, h_counts_[b], h_cells_[b]});
-      off += h_counts_[b];
-    }
-    std::stable_sort(lv.begin(), lv.end(),
-                     [](const Launch& x, const Launch& y) { return x.cells > y.cells; });
-    const int ns = std::min<int>((int)lv.size(), std::max(1, level_streams));
-    if (ns > 1) HIP_TRY(hipEventRecord(fork_ev_, stream_));
-    for (int k = 1; k < ns; ++k) HIP_TRY(hipStreamWaitEvent(side_[k - 1], fork_ev_, 0));
-    for (size_t i = 0; i < lv.size(); ++i) {
-      const Launch& w = lv[i];
-      hipStream_t s = (i % ns) == 0 ? stream_ : side_[i % ns - 1];
-      if (profile) HIP_TRY(hipEventRecord(prof_ev_[2 * prof.size()], s));
-      if (w.cls == kClassLane) {
-        launch_leaf(la, (int64_t)w.cnt, s);
-      } else {
-        da.tasks = tasks_.as<uint64_t>() + w.off;
-        da.ntasks = (int64_t)w.cnt;
-        if (w.cls >= kFirstXLClass) {
-          const int bw = xl_bw(w.cls - kFirstXLClass);
-          da.xl_off = xl_off_.as<int64_t>() + xl_first[w.bucket];
-          da.xl_ticket = tickets_.as<unsigned int>() + w.bucket;
-          da.xl_epoch = ++xl_epoch_ ? xl_epoch_ : ++xl_epoch_;  // never 0
-          da.xl_nbmax = (maxn + bw - 1) / bw;
-        }
-        const int lds = std::max(64, (h_maxlds_[w.bucket] + 15) & ~15);
-        if (!launch_forest(w.cls, da, lds, s)) {
-          set_error("run_batch: no kernel for class " + std::to_string(w.cls));
-          return ERR_INVALID_ARG;
-        }
+  hipStream_t sl[kMaxLevelStreams];
+  const int ns = std::min<int>((int)lv.size(), lane_streams(lane, sl));
+  if (ns > 1) HIP_TRY(hipEventRecord(fork_ev_, sl[0]));
+  for (int k = 1; k < ns; ++k) HIP_TRY(hipStreamWaitEvent(sl[k], fork_ev_, 0));
+  for (size_t i = 0; i < lv.size(); ++i) {
+    const Launch& w = lv[i];
+    Group& g = *w.g;
+    hipStream_t s = sl[i % ns];
+    if (prof) HIP_TRY(hipEventRecord(prof_ev_[2 * prof->size()], s));
+    if (w.cls == kClassLane) {
+      LeafArgs la;
+      la.pool = view();
+      la.pairs = g.pairs.as<PairMeta>();
+      la.line_start = g.lines_d.as<int64_t>();
+      la.npairs = g.npairs;
+      la.td = g.td.as<int32_t>();
+      launch_leaf(la, (int64_t)w.cnt, s);
+    } else {
+      DPArgs da;
+      da.pool = view();
+      da.pairs = g.pairs.as<PairMeta>();
+      da.td = g.td.as<int32_t>();
+      da.tasks = g.tasks.as<uint64_t>() + w.off;
+      da.ntasks = (int64_t)w.cnt;
+      da.xl_words = g.xl_words.as<unsigned long long>();
+      da.xl_abort = g.abort.as<int32_t>();
+      da.xl_off = nullptr;
+      da.xl_ticket = nullptr;
+      da.xl_epoch = 0;
+      da.xl_nbmax = 1;
+      if (w.cls >= kFirstXLClass) {
+        const int bw = xl_bw(w.cls - kFirstXLClass);
+        da.xl_off = g.xl_off.as<int64_t>() + g.xl_first[w.bucket];
+        da.xl_ticket = g.tickets.as<unsigned int>() + w.bucket;
+        // never 0 (a zeroed word) nor 0xFFFFFFFF (the reads' compare value)
+        if (++xl_epoch_ == 0xFFFFFFFFu) xl_epoch_ = 1;
+        da.xl_epoch = xl_epoch_;
+        da.xl_nbmax = (g.maxn + bw - 1) / bw;
       }
-      if (profile) {
-        HIP_TRY(hipEventRecord(prof_ev_[2 * prof.size() + 1], s));
-        prof.emplace_back(w.cls, w.bucket);
+      const int lds = std::max(64, (g.h_maxlds[w.bucket] + 15) & ~15);
+      if (!launch_forest(w.cls, da, lds, s)) {
+        set_error("run_batch: no kernel for class " + std::to_string(w.cls));
+        return ERR_INVALID_ARG;
       }
-      ++st.launches;
     }
-    for (int k = 1; k < ns; ++k) {
-      HIP_TRY(hipEventRecord(join_ev_[k - 1], side_[k - 1]));
-      HIP_TRY(hipStreamWaitEvent(stream_, join_ev_[k - 1], 0));
+    if (prof) {
+      HIP_TRY(hipEventRecord(prof_ev_[2 * prof->size() + 1], s));
+      prof->emplace_back(w.cls, w.bucket);
     }
+    ++g.st.launches;
+  }
+  for (int k = 1; k < ns; ++k) {
+    HIP_TRY(hipEventRecord(join_ev_[k - 1], sl[k]));
+    HIP_TRY(hipStreamWaitEvent(sl[0], join_ev_[k - 1], 0));
   }
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ev_[2], stream_));
-  launch_gather_dist(pa.pairs, da.td, npairs, dist_.as<int32_t>(), stream_);
+  return OK;
+}
+
+int Context::finish_async(Group& g, int lane) {
+  hipStream_t ls = lane_stream(lane);
+  if (g.pin_cap < g.npairs) {
+    if (g.h_dist_pin) HIP_TRY(hipHostFree(g.h_dist_pin));
+    g.h_dist_pin = nullptr;
+    g.pin_cap = 0;
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.h_dist_pin),
+                          sizeof(int32_t) * (g.npairs + 8), hipHostMallocDefault));
+    g.pin_cap = g.npairs;
+  }
+  if (!g.done_ev) HIP_TRY(hipEventCreateWithFlags(&g.done_ev, hipEventDisableTiming));
+  launch_gather_dist(g.pairs.as<PairMeta>(), g.td.as<int32_t>(), g.npairs, g.dist.as<int32_t>(),
+                     ls);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(dist, dist_.p, sizeof(int32_t) * npairs, hipMemcpyDeviceToHost, stream_));
-  int32_t aborted = 0;
-  if (nxl > 0)
-    HIP_TRY(hipMemcpyAsync(&aborted, abort_.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+  HIP_TRY(hipMemcpyAsync(g.h_dist_pin, g.dist.p, sizeof(int32_t) * g.npairs,
+                         hipMemcpyDeviceToHost, ls));
+  HIP_TRY(hipMemcpyAsync(g.h_dist_pin + g.npairs, g.abort.p, sizeof(int32_t) * 8,
+                         hipMemcpyDeviceToHost, ls));
+  HIP_TRY(hipEventRecord(g.done_ev, ls));
+  return OK;
+}
+
+int Context::poll(Group& g, bool wait, int32_t* dist) {
+  hipError_t e = wait ? hipEventSynchronize(g.done_ev) : hipEventQuery(g.done_ev);
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();
+    return ERR_BUSY;
+  }
+  HIP_TRY(e);
+  std::copy(g.h_dist_pin, g.h_dist_pin + g.npairs, dist);
+  if (g.nxl > 0 && g.h_dist_pin[g.npairs]) return xl_timeout(g.h_dist_pin + g.npairs);
+  return OK;
+}
+
+// The abort record of a column-blocked launch: {flag, claimed, ticket,
+// kind << 24 | row, word low, word high, epoch}.
+int xl_timeout(const int32_t* rec) {
+  set_error("run_batch: a column-blocked table stopped waiting for its neighbour block (ticket " +
+            std::to_string(rec[2]) + ", " + ((rec[3] >> 24) ? "import" : "carry") + " of row " +
+            std::to_string(rec[3] & 0xFFFFFF) + ", read " + std::to_string((uint32_t)rec[5]) +
+            ":" + std::to_string(rec[4]) + ", epoch " + std::to_string((uint32_t)rec[6]) + ")");
+  return ERR_TIMEOUT;
+}
+
+int Context::sync() {
+  HIP_TRY(hipStreamSynchronize(stream_));
+  return OK;
+}
+
+int Context::finish(Group& g, int32_t* dist, int32_t* const* td_out) {
+  launch_gather_dist(g.pairs.as<PairMeta>(), g.td.as<int32_t>(), g.npairs, g.dist.as<int32_t>(),
+                     stream_);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(dist, g.dist.p, sizeof(int32_t) * g.npairs, hipMemcpyDeviceToHost,
+                         stream_));
+  int32_t aborted[8] = {0};
+  if (g.nxl > 0)
+    HIP_TRY(hipMemcpyAsync(aborted, g.abort.p, sizeof(aborted), hipMemcpyDeviceToHost, stream_));
   if (td_out) {
-    for (int32_t i = 0; i < npairs; ++i) {
+    for (int32_t i = 0; i < g.npairs; ++i) {
       if (!td_out[i]) continue;
-      const int64_t mn = (int64_t)meta_[xs[i]].n * meta_[ys[i]].n;
-      HIP_TRY(hipMemcpyAsync(td_out[i], da.td + pm[i].td_off, sizeof(int32_t) * mn,
+      const int64_t mn = (int64_t)meta_[g.xs[i]].n * meta_[g.ys[i]].n;
+      HIP_TRY(hipMemcpyAsync(td_out[i], g.td.as<int32_t>() + g.pm[i].td_off, sizeof(int32_t) * mn,
                              hipMemcpyDeviceToHost, stream_));
     }
   }
   HIP_TRY(hipStreamSynchronize(stream_));
-  if (aborted) {
-    set_error("run_batch: a column-blocked table stopped waiting for its neighbour block");
-    return ERR_TIMEOUT;
+  if (aborted[0]) return xl_timeout(aborted);
+  return OK;
+}
+
+int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int32_t* dist,
+                       int32_t* const* td_out, BatchStats* stats) {
+  std::lock_guard<std::mutex> lk(mu_);
+  const auto t_start = std::chrono::steady_clock::now();
+  if (npairs < 0 || (npairs > 0 && (!xs || !ys || !dist))) {
+    set_error("run_batch: null argument");
+    return ERR_INVALID_ARG;
   }
+  if (npairs == 0) {
+    if (stats) *stats = BatchStats{};
+    return OK;
+  }
+  Group& g = batch_;
+  int st = plan(g, npairs, xs, ys);
+  if (st != OK) return st;
+  std::vector<std::pair<int, int>> prof;  // (class, bucket or -1) per timed launch
+  Group* one = &g;
+  while (g.next_level < g.nlevels) {
+    st = launch_levels(&one, 1, profile ? &prof : nullptr);
+    if (st != OK) return st;
+  }
+  HIP_TRY(hipEventRecord(ev_[2], stream_));
+  st = finish(g, dist, td_out);
+  if (st != OK) return st;
   if (profile) {
     for (auto& cp : class_profile) cp = ClassProfile{};
     for (size_t i = 0; i < prof.size(); ++i) {
@@ -547,19 +684,18 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
       cp.launches += 1;
       cp.ms += ms;
       const int b = prof[i].second;
-      cp.tables += b >= 0 ? (int64_t)h_counts_[b] : leaf_tables;
-      cp.cells += b >= 0 ? (int64_t)h_cells_[b] : leaf_cells;
+      cp.tables += b >= 0 ? (int64_t)g.h_counts[b] : g.leaf_tables;
+      cp.cells += b >= 0 ? (int64_t)g.h_cells[b] : g.leaf_cells;
     }
   }
   float plan_ms = 0, dp_ms = 0;
   HIP_TRY(hipEventElapsedTime(&plan_ms, ev_[0], ev_[1]));
   HIP_TRY(hipEventElapsedTime(&dp_ms, ev_[1], ev_[2]));
-  st.plan_ms = plan_ms;
-  st.dp_ms = dp_ms;
-  st.levels = nlevels;
-  st.total_ms =
+  g.st.plan_ms = plan_ms;
+  g.st.dp_ms = dp_ms;
+  g.st.total_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-  if (stats) *stats = st;
+  if (stats) *stats = g.st;
   return OK;
 }
 

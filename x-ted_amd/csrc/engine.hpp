@@ -36,6 +36,10 @@ struct BatchStats {
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  // Fine-grained memory keeps device-scope loads and stores coherent across
+  // the XCDs inside a kernel (the words column-blocked workgroups exchange);
+  // ordinary allocations are only made coherent at kernel boundaries.
+  bool fine = false;
   int grow(size_t bytes, bool keep, hipStream_t s);
   void release();
   template <class T>
@@ -48,6 +52,35 @@ struct ClassProfile {
   int64_t tables = 0;
   int64_t cells = 0;
   double ms = 0;  // summed device time of the class's launches
+};
+
+// One planned set of pairs: its plan (tasks bucketed by level and class),
+// its TD arena and its position in the level loop. A batch is one group; the
+// server keeps several in flight, each at its own level.
+struct Group {
+  int32_t npairs = 0, nlevels = 0, maxn = 1;
+  int32_t next_level = 0;           // next level to launch
+  unsigned long long level_off = 0; // tasks of the levels already launched
+  int64_t nxl = 0, leaf_tables = 0, leaf_cells = 0, td_bytes = 0;
+  std::vector<int32_t> xs, ys;
+  std::vector<PairMeta> pm;
+  std::vector<int64_t> lines, xl_first;
+  std::vector<unsigned long long> h_counts, h_cells;
+  std::vector<int32_t> h_maxlds;
+  std::vector<uint8_t> h_tail;
+  DevBuf pairs, blk, counts, cells, cursor, maxlds, tickets, tasks, td, dist, lines_d, tail_lv,
+      xl_off, xl_total;
+  DevBuf abort{nullptr, 0, true}, xl_words{nullptr, 0, true};  // fine-grained
+  // finish_async: distances then the abort flag land in pinned memory
+  int32_t* h_dist_pin = nullptr;
+  int32_t pin_cap = 0;
+  hipEvent_t done_ev = nullptr;
+  BatchStats st;
+  Group() = default;
+  Group(const Group&) = delete;
+  Group& operator=(const Group&) = delete;
+  ~Group() { release(); }
+  void release();
 };
 
 struct HostTreeInfo {
@@ -79,6 +112,27 @@ class Context {
   int run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int32_t* dist,
                 int32_t* const* td_out, BatchStats* stats);
 
+  // The steps of run_batch, for callers that interleave several groups (the
+  // server). The caller holds mutex() across them.
+  // Plans pairs (xs[i], ys[i]) into g: device plan, one sync for the counts.
+  // lane: -1 = all streams (a batch); 0 .. kMaxLevelStreams-1 = server lanes,
+  // one stream each, whose ticks run independently of each other.
+  int plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys, int lane = -1);
+  // One tick: launches the next level of every group in gs (groups past
+  // their last level are skipped) and advances them. Asynchronous.
+  int launch_levels(Group* const* gs, int ng, std::vector<std::pair<int, int>>* prof,
+                    int lane = -1);
+  // Distances (and tables) of a group whose levels are all launched; syncs.
+  int finish(Group& g, int32_t* dist, int32_t* const* td_out);
+  std::mutex& mutex() { return mu_; }
+  int sync();  // waits for everything launched on the context's streams
+  // Enqueues the distance gather of a group whose levels are all launched;
+  // poll() returns ERR_BUSY until it has landed (wait = block for it).
+  int finish_async(Group& g, int lane = -1);
+  hipStream_t lane_stream(int lane) const;  // where a lane's ticks end
+  int lane_streams(int lane, hipStream_t* out) const;
+  int poll(Group& g, bool wait, int32_t* dist);
+
   // Tuning / test hooks -----------------------------------------------------
   ClassRule rule{kLdsPerWave, 1, 1 << 30, kLdsMaxTable, 8};
   int32_t lds_limit = 0;  // device's dynamic LDS per workgroup for the slots
@@ -99,9 +153,11 @@ class Context {
  private:
   int device_;
   hipStream_t stream_ = nullptr;
+  hipStream_t plan_stream_ = nullptr;
   hipStream_t side_[kMaxLevelStreams - 1] = {};
   hipEvent_t ev_[4] = {};
   hipEvent_t fork_ev_ = nullptr, join_ev_[kMaxLevelStreams - 1] = {};
+  hipEvent_t plan_done_ = nullptr;
   std::mutex mu_;
   // pool
   std::vector<TreeMeta> meta_;
@@ -110,16 +166,8 @@ class Context {
   DevBuf parent_, label_, orl_, kdep_, kdn_, lrank_, kr_, krkd_, krsize_, krleaf_, trees_;
   DevBuf skr_, skd_, ssize_, sleaf_, spre_, kdoff_, rtop_;
   DevBuf prep_scratch_, prep_off_;
-  // batch
-  DevBuf pairs_, blk_, counts_, cells_, cursor_, maxlds_, tasks_, td_, dist_;
-  DevBuf lines_;
-  // column-blocked tables: launch tickets, abort flag, per-task arena offsets,
-  // the arena of cross-workgroup words and the tag of the latest launch
-  DevBuf tickets_, abort_, xl_off_, xl_total_, xl_words_, tail_lv_;
-  std::vector<uint8_t> h_tail_;
-  uint32_t xl_epoch_ = 0;
-  std::vector<unsigned long long> h_counts_, h_cells_;
-  std::vector<int32_t> h_maxlds_;  // largest slot storage per (level, class) bucket
+  Group batch_;           // the group run_batch reuses
+  uint32_t xl_epoch_ = 0;  // tag of the latest column-blocked launch
   std::vector<hipEvent_t> prof_ev_;
   int grow_pool(int64_t nodes);
   PoolView view() const;

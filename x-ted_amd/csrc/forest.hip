@@ -146,26 +146,68 @@ typedef int32_t int2u __attribute__((ext_vector_type(2), aligned(4)));
 // matches, so no fence is needed and nothing is cleared between launches.
 // Workgroups take (table, block) from a ticket counter in start order, so
 // a block only ever waits on blocks that are already running. Waits are
-// bounded: after kXLSpinLimit polls a workgroup raises the abort flag,
-// every waiting workgroup then stops waiting, and the call fails.
+// bounded: after kXLWaitTicks of the constant 100 MHz wall clock a
+// workgroup raises the abort flag (recording what it waited for), every
+// waiting workgroup then stops waiting, and the call fails.
 // ---------------------------------------------------------------------------
-constexpr int kXLSpinLimit = 1 << 22;
+constexpr uint64_t kXLWaitTicks = 1000000000ull;  // 10 s at 100 MHz
 
-__device__ __forceinline__ unsigned long long xl_load(const unsigned long long* p) {
+// Writes are atomic exchanges, performed where every XCD sees the same line.
+// A word is read twice at most: first with a plain load, issued rows ahead,
+// which is right whenever its epoch matches (a word is written once per
+// launch); a stale first read (the line can sit in the reader's XCD-local L2
+// from before the write, and polls would keep hitting that copy) is polled
+// again with atomic reads, which see the written value. An atomic read is a
+// compare-exchange against a value no word holds (epoch 0xFFFFFFFF is never
+// issued), which fails and returns the current value; an idempotent
+// or-with-0 would be turned back into a plain load by the compiler.
+constexpr unsigned long long kXLNever = ~0ull;
+__device__ __forceinline__ unsigned long long xl_peek(const unsigned long long* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void xl_store(unsigned long long* p, uint32_t epoch, int32_t v) {
-  __hip_atomic_store(p, ((unsigned long long)epoch << 32) | (uint32_t)v, __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ unsigned long long xl_load(const unsigned long long* p) {
+  unsigned long long cur = kXLNever;
+  (void)__hip_atomic_compare_exchange_strong(const_cast<unsigned long long*>(p), &cur, kXLNever,
+                                             __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+  return cur;
 }
+__device__ __forceinline__ void xl_store(unsigned long long* p, uint32_t epoch, int32_t v) {
+  (void)__hip_atomic_exchange(p, ((unsigned long long)epoch << 32) | (uint32_t)v,
+                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int32_t xl_flag(int32_t* f) {
+  int32_t cur = -1;  // the flag is 0 or 1
+  (void)__hip_atomic_compare_exchange_strong(f, &cur, -1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+  return cur;
+}
+
 // Lanes with `need` poll *p until it carries `epoch` (w is the first read).
+// abort_flag[0] is the flag; the first workgroup to give up records
+// what = (ticket, kind << 24 | row, word low, word high) in abort_flag[2..5].
 __device__ __forceinline__ int32_t xl_wait(const unsigned long long* p, unsigned long long w,
-                                           bool need, uint32_t epoch, int32_t* abort_flag) {
+                                           bool need, uint32_t epoch, int32_t* abort_flag,
+                                           int32_t ticket, int32_t what) {
+  uint64_t t0 = 0;
   for (int spins = 0; __ballot(need && (uint32_t)(w >> 32) != epoch); ++spins) {
-    if ((spins & 63) == 63) {
-      if (spins > kXLSpinLimit) __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-      if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    if ((spins & 63) == 0) {
+      const uint64_t now = wall_clock64();
+      if (spins == 0) t0 = now;
+      else if (now - t0 > kXLWaitTicks) {
+        const bool stale = need && (uint32_t)(w >> 32) != epoch;
+        const unsigned long long first = __ballot(stale);
+        if ((int)(threadIdx.x & 63) == __ffsll((long long)first) - 1 &&
+            atomicCAS(abort_flag + 1, 0, 1) == 0) {
+          abort_flag[2] = ticket;
+          abort_flag[3] = what;
+          abort_flag[4] = (int32_t)(uint32_t)w;
+          abort_flag[5] = (int32_t)(w >> 32);
+          abort_flag[6] = (int32_t)epoch;
+        }
+        (void)__hip_atomic_exchange(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (xl_flag(abort_flag)) break;
     }
     __builtin_amdgcn_s_sleep(2);
     if (need && (uint32_t)(w >> 32) != epoch) w = xl_load(p);
@@ -228,12 +270,13 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   __shared__ int32_t xsh[kXL ? kMaxImports + 1 : 1];  // import flags, then the ticket
 
   int64_t tix = blockIdx.x;
-  int32_t blk = 0;
+  int32_t blk = 0, ticket = 0;
   if constexpr (kXL) {
     if (threadIdx.x == 0) xsh[kMaxImports] = (int32_t)atomicAdd(A.xl_ticket, 1u);
     for (int k = threadIdx.x; k < kMaxImports; k += 64 * W) xsh[k] = 0;
     __syncthreads();
     const int32_t tk = xsh[kMaxImports];
+    ticket = tk;
     tix = tk / A.xl_nbmax;
     blk = tk % A.xl_nbmax;
   }
@@ -408,10 +451,11 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   load_meta(0, mlab, mrf, mslot, mrid);
   if (nchunks > 1) load_meta(1, nlab, nrf, nslot, nrid);
 
-  // Column-blocked form: the upstream carry (cq) and this block's imports
-  // (iq, wave 0 only) of row k, validated, sit in ring slot k % S. They are
-  // read PD rows ahead and checked at the end of the row that read them, so
-  // a block runs at least PD rows behind the one before it.
+  // Column-blocked form: the upstream carry (cr) and this block's imports
+  // (ir, wave 0 only) of row k are read into ring slot k % S when row k-PD
+  // starts and checked when row k uses them, so reads have PD rows to land;
+  // a word not yet written is polled again then. A read not wanted holds
+  // kXLNever's complement tagged with the epoch, which checks as valid.
   unsigned long long* xw = A.xl_words;
   int64_t cin_off = 0, cout_off = 0, exp_off = 0;
   if constexpr (kXL) {
@@ -420,43 +464,27 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     cout_off = (int64_t)blk * nrows;
     exp_off = (int64_t)(nb - 1) * nrows;
   }
-  int32_t cq[kXL ? S : 1], iq[kXL ? S : 1][KQ];
-  unsigned long long cw = 0, iw[KQ];
-  bool iwant[KQ];
-  int32_t irid = -1;
+  const unsigned long long xvalid = (unsigned long long)A.xl_epoch << 32;
+  unsigned long long cr[kXL ? S : 1], ir[kXL ? S : 1][KQ];
+  int32_t irid[kXL ? S : 1];
   // Export word of saved row rid for block bb at depth index kx.
   auto xword = [&](int32_t rid, int32_t bb, int32_t kx) {
     return xw + exp_off + ((int64_t)rid * nb + bb) * kds + kx;
   };
-  auto xl_issue = [&](int32_t k, int32_t rid) {  // row k of the table
+  auto xl_issue = [&](int32_t k, int32_t rid, int slot) {  // row k of the table
     if constexpr (kXL) {
-      if (blk > 0 && k < nrows) cw = xl_load(xw + cin_off + k);
-      irid = rid;
+      cr[slot] = (blk > 0 && k < nrows) ? xl_peek(xw + cin_off + k) : xvalid;
+      irid[slot] = k < nrows ? rid : -1;
 #pragma unroll
       for (int q = 0; q < KQ; ++q) {
-        iwant[q] = wave == 0 && impl[q] && rid >= 0 && k < nrows;
-        if (iwant[q]) iw[q] = xl_load(xword(rid, blk, lane + 64 * q));
-      }
-    }
-  };
-  auto xl_finish = [&](int32_t k, int u) {
-    if constexpr (kXL) {
-      if (blk > 0 && k < nrows) cq[u] = xl_wait(xw + cin_off + k, cw, true, A.xl_epoch, A.xl_abort);
-      if (wave == 0) {
-#pragma unroll
-        for (int q = 0; q < KQ; ++q)
-          iq[u][q] = xl_wait(xword(max(irid, 0), blk, lane + 64 * q), iw[q], iwant[q], A.xl_epoch,
-                             A.xl_abort);
+        const bool want = wave == 0 && impl[q] && rid >= 0 && k < nrows;
+        ir[slot][q] = want ? xl_peek(xword(rid, blk, lane + 64 * q)) : xvalid;
       }
     }
   };
   if constexpr (kXL) {
 #pragma unroll
-    for (int k = 0; k < PD; ++k) {
-      if (k >= nrows) break;
-      xl_issue(k, __builtin_amdgcn_readlane(mrid, k));
-      xl_finish(k, k);
-    }
+    for (int k = 0; k < PD; ++k) xl_issue(k, k < nrows ? __builtin_amdgcn_readlane(mrid, k) : -1, k);
   }
 
   for (int32_t q = 0; q < nchunks; ++q) {
@@ -476,7 +504,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           const int32_t ia = i + PD;
           const int32_t rid = ia < 64 ? __builtin_amdgcn_readlane(mrid, ia)
                                       : __builtin_amdgcn_readlane(nrid, ia - 64);
-          xl_issue(64 * q + ia, rid);
+          xl_issue(64 * q + ia, rid, (u + PD) % S);
         }
 
         const int32_t labr = __builtin_amdgcn_readlane(mlab, i);
@@ -529,7 +557,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
         // totals of this workgroup's earlier waves.
         int32_t carry = (rmax - r) + 1;
         if constexpr (kXL) {
-          if (blk > 0) carry = cq[u];
+          if (blk > 0) {
+            const int32_t k = 64 * q + i;
+            carry = xl_wait(xw + cin_off + k, cr[u], true, A.xl_epoch, A.xl_abort, ticket, k);
+          }
         }
         if constexpr (W > 1) {
           int32_t* xt = xtot[r & 1];
@@ -576,9 +607,14 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
             // Imported far values of this saved row; far values later
             // blocks import from this one.
             if (wave == 0) {
+              const int32_t k = 64 * q + i;
 #pragma unroll
-              for (int q2 = 0; q2 < KQ; ++q2)
-                if (impl[q2]) wrow_ptr[wb + 1 + lane + 64 * q2] = (uint16_t)iq[u][q2];
+              for (int q2 = 0; q2 < KQ; ++q2) {
+                const int32_t v = xl_wait(xword(max(irid[u], 0), blk, lane + 64 * q2), ir[u][q2],
+                                          impl[q2], A.xl_epoch, A.xl_abort, ticket,
+                                          (1 << 24) | k);
+                if (impl[q2]) wrow_ptr[wb + 1 + lane + 64 * q2] = (uint16_t)v;
+              }
             }
             const int32_t rid = __builtin_amdgcn_readlane(mrid, i);
 #pragma unroll
@@ -608,7 +644,6 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
               g ? __builtin_amdgcn_readlane(h[jl - E] - negt[jl - E], 63) : left_in;
           fpr0[g] = wave_shr1(h[jl] - negt[jl], fill);
         }
-        if constexpr (kXL) xl_finish(64 * q + i + PD, (u + PD) % S);
       }
     }
     mlab = nlab;
@@ -730,7 +765,8 @@ hipError_t forest_init_attributes(int limit) {
   if ((e = allow_lds<8, 4, true>(limit)) != hipSuccess) return e;
   if ((e = allow_lds<4, 4, true>(limit)) != hipSuccess) return e;
   if ((e = allow_lds<2, 4, true>(limit)) != hipSuccess) return e;
-  return allow_lds<2, 2, true>(limit);
+  if ((e = allow_lds<2, 2, true>(limit)) != hipSuccess) return e;
+  return allow_lds<2, 1, true>(limit);
 }
 
 // Launches one (class, task list). Returns false for a class without a kernel.
@@ -753,6 +789,7 @@ bool launch_forest(int cls, const DPArgs& a, int lds_bytes, hipStream_t s) {
     case 2: launch_xl<4, 4>(a, lds_bytes, s); return true;
     case 3: launch_xl<2, 4>(a, lds_bytes, s); return true;
     case 4: launch_xl<2, 2>(a, lds_bytes, s); return true;
+    case 5: launch_xl<2, 1>(a, lds_bytes, s); return true;
     default: return false;
   }
 }

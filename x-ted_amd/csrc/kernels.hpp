@@ -53,7 +53,7 @@ struct PlanArgs {
   int32_t* maxlds;             // nlevels*kNumClasses: largest slot storage per bucket
   uint64_t* tasks;
   unsigned int* tickets;       // nlevels*kNumClasses: column-blocked launch counters
-  int32_t* abort_flag;         // column-blocked waits gave up
+  int32_t* abort_flag;         // 8 ints: column-blocked waits gave up, and what they waited for
 };
 // Blocks a pair occupies in the plan kernels (same formula on both sides).
 inline int64_t plan_blocks(int32_t Kx, int32_t Ky, int32_t chunk) {
@@ -100,7 +100,7 @@ struct DPArgs {
   unsigned long long* xl_words;  // cross-workgroup words of every such task
   const int64_t* xl_off;         // this launch's tasks' offsets into xl_words
   unsigned int* xl_ticket;       // (task, block) counter of this launch, zeroed
-  int32_t* xl_abort;             // raised when a wait gives up
+  int32_t* xl_abort;             // [0] raised when a wait gives up, [1..6] what it waited for
   uint32_t xl_epoch;             // tag of this launch's words (never 0)
   int32_t xl_nbmax;              // workgroups per task in the grid
 };

@@ -26,8 +26,8 @@ ERR_CANCELLED = 10
 
 CLASS_AUTO = -1
 CLASS_LANE = 0
-NUM_CLASSES = 32
-XL_BW = (4096, 2048, 1024, 512, 256)  # column-block widths of classes 27..31
+NUM_CLASSES = 33
+XL_BW = (4096, 2048, 1024, 512, 256, 128)  # column-block widths of classes 27..32
 
 
 def class_name(cls):
@@ -39,7 +39,7 @@ def class_name(cls):
     if 1 <= cls <= 25:
         wi, ci = divmod(cls - 1, 5)
         return f"rowscan_w{1 << wi}_c{1 << ci}"
-    if 27 <= cls < 32:
+    if 27 <= cls < NUM_CLASSES:
         return f"xl_bw{XL_BW[cls - 27]}"
     return "too_large"
 
