@@ -221,7 +221,7 @@ def main():
     ap.add_argument("--no-xl", action="store_true", help="skip the 20,000-node pair")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="also run the serving leg (Poisson arrivals) for this long")
-    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-trees", type=int, default=10000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the "

@@ -119,6 +119,10 @@ int xted_force_class(xted_context* ctx, int32_t cls, int32_t lane_enable);
  * there the rule takes the fewest waves, up to 16 columns per lane and slot
  * storage up to the device limit. 0 turns this off. */
 int xted_set_tail_rule(xted_context* ctx, int32_t tail_pairs);
+/* Column-blocked tables with at most seq_rows rows are swept block after
+ * block by one workgroup (no waits between workgroups); taller ones run one
+ * workgroup per block, chained per row. */
+int xted_set_xl_rule(xted_context* ctx, int32_t seq_rows);
 /* Streams (1 to 4) that the launches of one level run on concurrently. The
  * reference gives each class kernel its own stream but waits for each
  * launch before the next (X-TED_GPU/parallel_X_TED.cu:433-438). Results do

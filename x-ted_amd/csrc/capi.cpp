@@ -213,6 +213,15 @@ int xted_force_class(xted_context* ctx, int32_t cls, int32_t lane_enable) {
   return OK;
 }
 
+int xted_set_xl_rule(xted_context* ctx, int32_t seq_rows) {
+  if (!ctx || seq_rows < 0) {
+    set_error("xted_set_xl_rule: invalid argument");
+    return ERR_INVALID_ARG;
+  }
+  ctx->ctx.rule.xl_seq_rows = seq_rows;
+  return OK;
+}
+
 int xted_set_tail_rule(xted_context* ctx, int32_t tail_pairs) {
   if (!ctx || tail_pairs < 0) {
     set_error("xted_set_tail_rule: invalid argument");

@@ -93,6 +93,11 @@ constexpr int kNumClasses = kFirstXLClass + kNumXLForms;  // 33
 XTED_HD inline int xl_waves(int k) { return k <= 1 ? 8 : k == 2 ? 4 : 2; }
 XTED_HD inline int xl_cpl(int k) { return k == 0 ? 8 : k <= 3 ? 4 : k == 4 ? 2 : 1; }
 XTED_HD inline int xl_bw(int k) { return 64 * xl_waves(k) * xl_cpl(k); }  // 4096 .. 128
+// Column-blocked tables with at most this many rows are swept block after
+// block by one workgroup (forms of at most 1024 columns), with no waits
+// between workgroups; taller ones pipeline one workgroup per block.
+constexpr int32_t kXLSeqRows = 256;  // default of ClassRule::xl_seq_rows
+constexpr int kFirstSeqXLForm = 2;
 // Far values a block takes from earlier blocks are indexed by the nesting
 // depth of their right path below the column keyroot: at most this many.
 constexpr int kMaxImports = 256;
@@ -115,6 +120,7 @@ struct ClassRule {
   int32_t wide_cols;
   int32_t lds_max;       // slot bytes one workgroup may hold (device limit)
   int32_t max_cpl;       // most columns per lane before a table gets more waves
+  int32_t xl_seq_rows = kXLSeqRows;  // column-blocked tables this short: one workgroup
 };
 
 XTED_HD inline int ilog2_ceil(int v) {
@@ -145,7 +151,8 @@ XTED_HD inline int table_class(int a, int b, int64_t lds_need, int kd_r, int kd_
   const int cpl_cap = rule.max_cpl < 8 ? rule.max_cpl : 8;
   if (cols > 64 * cpl_cap * kMaxRowWaves || lds_need > rule.lds_max) {
     if (kd_c + 1 > kMaxImports) return kClassXL;
-    for (int k = 0; k < kNumXLForms; ++k) {
+    // Short tables take the forms one workgroup sweeps.
+    for (int k = rows <= rule.xl_seq_rows ? kFirstSeqXLForm : 0; k < kNumXLForms; ++k) {
       const int64_t n = xl_lds_need(kd_r, kd_c, leaves_c, xl_bw(k));
       if (n <= rule.lds_max) {
         *need = n;

@@ -404,7 +404,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   pa.nlevels = nlevels;
   pa.lchunk_tables = plan_chunk;
   pa.rule = rule;
-  pa.rule_tail = ClassRule{lds_limit, 1, 1 << 30, lds_limit, 16};
+  pa.rule_tail = ClassRule{lds_limit, 1, 1 << 30, lds_limit, 16, rule.xl_seq_rows};
   pa.tail_level = g.tail_lv.as<uint8_t>();
   pa.force_class = force_class;
   pa.lane_enable = lane_enable;
@@ -551,6 +551,7 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<std::pair<int, 
       da.xl_ticket = nullptr;
       da.xl_epoch = 0;
       da.xl_nbmax = 1;
+      da.xl_seq_rows = rule.xl_seq_rows;
       if (w.cls >= kFirstXLClass) {
         const int bw = xl_bw(w.cls - kFirstXLClass);
         da.xl_off = g.xl_off.as<int64_t>() + g.xl_first[w.bucket];

@@ -36,10 +36,7 @@ struct BatchStats {
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
-  // Fine-grained memory keeps device-scope loads and stores coherent across
-  // the XCDs inside a kernel (the words column-blocked workgroups exchange);
-  // ordinary allocations are only made coherent at kernel boundaries.
-  bool fine = false;
+  bool fine = false;  // fine-grained allocation (hipDeviceMallocFinegrained)
   int grow(size_t bytes, bool keep, hipStream_t s);
   void release();
   template <class T>
@@ -70,7 +67,7 @@ struct Group {
   std::vector<uint8_t> h_tail;
   DevBuf pairs, blk, counts, cells, cursor, maxlds, tickets, tasks, td, dist, lines_d, tail_lv,
       xl_off, xl_total;
-  DevBuf abort{nullptr, 0, true}, xl_words{nullptr, 0, true};  // fine-grained
+  DevBuf abort, xl_words;
   // finish_async: distances then the abort flag land in pinned memory
   int32_t* h_dist_pin = nullptr;
   int32_t pin_cap = 0;
@@ -134,7 +131,7 @@ class Context {
   int poll(Group& g, bool wait, int32_t* dist);
 
   // Tuning / test hooks -----------------------------------------------------
-  ClassRule rule{kLdsPerWave, 1, 1 << 30, kLdsMaxTable, 8};
+  ClassRule rule{kLdsPerWave, 1, 1 << 30, kLdsMaxTable, 8, kXLSeqRows};
   int32_t lds_limit = 0;  // device's dynamic LDS per workgroup for the slots
   int32_t force_class = -1;
   int32_t lane_enable = 1;

@@ -151,6 +151,13 @@ typedef int32_t int2u __attribute__((ext_vector_type(2), aligned(4)));
 // waiting workgroup then stops waiting, and the call fails.
 // ---------------------------------------------------------------------------
 constexpr uint64_t kXLWaitTicks = 1000000000ull;  // 10 s at 100 MHz
+// A pipelined block checks its predecessor's progress every kXLEvery rows
+// (a power of two dividing 64).
+constexpr int32_t kXLEvery = 64;
+// Tables with at most xl_seq_rows rows are swept block after block by one
+// workgroup (forms of at most 1024 columns): their words stay inside that
+// workgroup (same CU, same L2), so they need no waits and no atomics; a
+// pipeline of workgroups pays off only when rows are many.
 
 // Writes are atomic exchanges, performed where every XCD sees the same line.
 // A word is read twice at most: first with a plain load, issued rows ahead,
@@ -172,9 +179,11 @@ __device__ __forceinline__ unsigned long long xl_load(const unsigned long long* 
                                              __HIP_MEMORY_SCOPE_AGENT);
   return cur;
 }
-__device__ __forceinline__ void xl_store(unsigned long long* p, uint32_t epoch, int32_t v) {
-  (void)__hip_atomic_exchange(p, ((unsigned long long)epoch << 32) | (uint32_t)v,
-                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void xl_store(unsigned long long* p, uint32_t epoch, int32_t v,
+                                         bool local) {
+  const unsigned long long w = ((unsigned long long)epoch << 32) | (uint32_t)v;
+  if (local) __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else (void)__hip_atomic_exchange(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ int32_t xl_flag(int32_t* f) {
   int32_t cur = -1;  // the flag is 0 or 1
@@ -209,17 +218,21 @@ __device__ __forceinline__ int32_t xl_wait(const unsigned long long* p, unsigned
       }
       if (xl_flag(abort_flag)) break;
     }
-    __builtin_amdgcn_s_sleep(2);
+    if (spins < 4) __builtin_amdgcn_s_sleep(8);
+    else __builtin_amdgcn_s_sleep(32);
     if (need && (uint32_t)(w >> 32) != epoch) w = xl_load(p);
   }
   return (int32_t)(uint32_t)w;
 }
 
-// Arena words of one column-blocked task: (nb-1)*nrows carries, then
-// nsaved*nb*kds export words (saved row, destination block, depth).
+// Arena words of one column-blocked task: nb-1 runs of carries (one per
+// row, each run padded to whole 64-byte lines), then for every saved row and
+// destination block a line-padded run of kds export words (by depth). A
+// line thus only holds words its producer writes together, in order.
 struct XLGeom {
   int32_t nrows, nb, nsaved, kds;
 };
+__device__ __forceinline__ int32_t xl_pad(int32_t w) { return (w + 7) & ~7; }
 __device__ __forceinline__ XLGeom xl_geom(const PoolView& P, const PairMeta& pm, uint64_t task,
                                           int bw) {
   const TreeMeta tx = P.trees[pm.tx], ty = P.trees[pm.ty];
@@ -237,7 +250,7 @@ __device__ __forceinline__ XLGeom xl_geom(const PoolView& P, const PairMeta& pm,
   return g;
 }
 __device__ __forceinline__ int64_t xl_words(const XLGeom& g) {
-  return (int64_t)(g.nb - 1) * g.nrows + (int64_t)g.nsaved * g.nb * g.kds;
+  return (int64_t)(g.nb - 1) * xl_pad(g.nrows) + (int64_t)g.nsaved * g.nb * xl_pad(g.kds);
 }
 
 // ---------------------------------------------------------------------------
@@ -263,6 +276,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   constexpr int PD = S - 1;  // rows of TD prefetched
   constexpr int BW = 64 * W * C;                    // columns of one block
   constexpr int KQ = kXL ? kMaxImports / 64 : 1;    // import entries per lane
+  constexpr bool kSeqForm = kXL && W * C <= 16;      // may sweep all blocks itself
   extern __shared__ uint16_t slots[];
   // Per-wave scan totals, double-buffered by row parity: a wave may write
   // row r-1's total while a slower wave still reads row r's.
@@ -311,9 +325,20 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   const int32_t base = kdepR[r0];
   const int32_t orl_c0 = cmax - 1;
   int32_t nb = 1;
+  bool seq = false;  // one workgroup sweeps every block of the table
   if constexpr (kXL) {
     nb = (ncols + BW - 1) / BW;
-    if (blk >= nb) return;
+    seq = kSeqForm && nrows <= A.xl_seq_rows;
+    if (blk >= nb || (seq && blk > 0)) return;
+  }
+  do {
+  if constexpr (kSeqForm) {
+    if (blk > 0 && seq) {
+      // The previous block's words are written; the import flags restart.
+      __syncthreads();
+      for (int k = threadIdx.x; k < kMaxImports; k += 64 * W) xsh[k] = 0;
+      __syncthreads();
+    }
   }
   const int32_t tb0 = blk * BW;  // first t of this block (0 on chip)
   // A slot row holds the compact far columns of this block (leaves but the
@@ -451,38 +476,72 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   load_meta(0, mlab, mrf, mslot, mrid);
   if (nchunks > 1) load_meta(1, nlab, nrf, nslot, nrid);
 
-  // Column-blocked form: the upstream carry (cr) and this block's imports
-  // (ir, wave 0 only) of row k are read into ring slot k % S when row k-PD
-  // starts and checked when row k uses them, so reads have PD rows to land;
-  // a word not yet written is polled again then. A read not wanted holds
-  // kXLNever's complement tagged with the epoch, which checks as valid.
+  // Column-blocked form: the upstream carry (crv) and this block's imports
+  // (irv, wave 0 only) of row k are read into ring slot k % S when row k-PD
+  // starts, with one bit each saying whether the read already carried this
+  // launch's epoch (crok, irok); when row k uses them a stale one is polled
+  // again. Reads not wanted count as valid.
   unsigned long long* xw = A.xl_words;
   int64_t cin_off = 0, cout_off = 0, exp_off = 0;
+  const int32_t kdsp = (kds + 7) & ~7;
   if constexpr (kXL) {
+    const int64_t cstride = (nrows + 7) & ~7;
     xw += A.xl_off[tix];
-    cin_off = (int64_t)(blk - 1) * nrows;
-    cout_off = (int64_t)blk * nrows;
-    exp_off = (int64_t)(nb - 1) * nrows;
+    cin_off = (int64_t)(blk - 1) * cstride;
+    cout_off = (int64_t)blk * cstride;
+    exp_off = (int64_t)(nb - 1) * cstride;
   }
-  const unsigned long long xvalid = (unsigned long long)A.xl_epoch << 32;
-  unsigned long long cr[kXL ? S : 1], ir[kXL ? S : 1][KQ];
-  int32_t irid[kXL ? S : 1];
+  int32_t crv[kXL ? S : 1], irv[kXL ? S : 1][KQ], irid[kXL ? S : 1];
+  uint32_t crok = 0, irok[kXL ? S : 1];
   // Export word of saved row rid for block bb at depth index kx.
   auto xword = [&](int32_t rid, int32_t bb, int32_t kx) {
-    return xw + exp_off + ((int64_t)rid * nb + bb) * kds + kx;
+    return xw + exp_off + ((int64_t)rid * nb + bb) * kdsp + kx;
   };
   auto xl_issue = [&](int32_t k, int32_t rid, int slot) {  // row k of the table
     if constexpr (kXL) {
-      cr[slot] = (blk > 0 && k < nrows) ? xl_peek(xw + cin_off + k) : xvalid;
+      bool ok = true;
+      if (blk > 0 && k < nrows) {
+        const unsigned long long w = xl_peek(xw + cin_off + k);
+        crv[slot] = (int32_t)(uint32_t)w;
+        ok = (uint32_t)(w >> 32) == A.xl_epoch;
+      }
+      crok = ok ? (crok | (1u << slot)) : (crok & ~(1u << slot));
       irid[slot] = k < nrows ? rid : -1;
+      uint32_t bits = 0;
 #pragma unroll
       for (int q = 0; q < KQ; ++q) {
-        const bool want = wave == 0 && impl[q] && rid >= 0 && k < nrows;
-        ir[slot][q] = want ? xl_peek(xword(rid, blk, lane + 64 * q)) : xvalid;
+        bool okq = true;
+        if (wave == 0 && impl[q] && rid >= 0 && k < nrows) {
+          const unsigned long long w = xl_peek(xword(rid, blk, lane + 64 * q));
+          irv[slot][q] = (int32_t)(uint32_t)w;
+          okq = (uint32_t)(w >> 32) == A.xl_epoch;
+        }
+        bits |= okq ? (1u << q) : 0u;
+      }
+      irok[slot] = bits;
+    }
+  };
+  // Before the reads of rows k+PD .. k+PD+kXLEvery-1 are issued, the block
+  // before this one has finished every row up to the end of the line holding
+  // the last of them (its later rows' lines are not touched yet), and so have
+  // the exporters, which run ahead of it: those plain reads find their lines
+  // complete. One atomic poll per kXLEvery rows.
+  // The poll for row k's check is read 4 rows ahead (tw), so its round trip
+  // overlaps those rows.
+  unsigned long long tw = 0;
+  auto xl_throttle_row = [&](int32_t k) { return min((k + PD + kXLEvery - 1) | 7, nrows - 1); };
+  auto xl_throttle_word = [&](int32_t k) { return xw + cin_off + xl_throttle_row(k); };
+  auto xl_throttle = [&](int32_t k, bool early) {
+    if constexpr (kXL) {
+      if (blk > 0 && !seq) {
+        const int32_t t = xl_throttle_row(k);
+        (void)xl_wait(xl_throttle_word(k), early ? tw : xl_load(xl_throttle_word(k)), true,
+                      A.xl_epoch, A.xl_abort, ticket, t);
       }
     }
   };
   if constexpr (kXL) {
+    xl_throttle(0, false);
 #pragma unroll
     for (int k = 0; k < PD; ++k) xl_issue(k, k < nrows ? __builtin_amdgcn_readlane(mrid, k) : -1, k);
   }
@@ -501,6 +560,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
         const int32_t r = rmax - 1 - (64 * q + i);
         load_row(max(r - PD, r0), tdq[(u + PD) % S]);
         if constexpr (kXL) {
+          constexpr int32_t kM = kXLEvery - 1, kHalf = kXLEvery / 2;
+          if ((i & kM) == 0 && (i > 0 || q > 0)) xl_throttle(64 * q + i, true);
+          if ((i & kM) == kHalf && blk > 0 && !seq)
+            tw = xl_load(xl_throttle_word(64 * q + i + kHalf));
           const int32_t ia = i + PD;
           const int32_t rid = ia < 64 ? __builtin_amdgcn_readlane(mrid, ia)
                                       : __builtin_amdgcn_readlane(nrid, ia - 64);
@@ -559,7 +622,9 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
         if constexpr (kXL) {
           if (blk > 0) {
             const int32_t k = 64 * q + i;
-            carry = xl_wait(xw + cin_off + k, cr[u], true, A.xl_epoch, A.xl_abort, ticket, k);
+            carry = (crok >> u) & 1u ? crv[u]
+                                     : xl_wait(xw + cin_off + k, 0ull, true, A.xl_epoch,
+                                               A.xl_abort, ticket, k);
           }
         }
         if constexpr (W > 1) {
@@ -572,7 +637,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           // The next block's carry: this block's last wave, carry-in
           // combined with its own total.
           if (blk + 1 < nb && wave == W - 1 && lane == 0)
-            xl_store(xw + cout_off + (64 * q + i), A.xl_epoch, min(carry, carry_w));
+            xl_store(xw + cout_off + (64 * q + i), A.xl_epoch, min(carry, carry_w), seq);
         }
         // h becomes G - t; Fpm = G + 1 - t = h + 1.
 #pragma unroll
@@ -610,10 +675,14 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
               const int32_t k = 64 * q + i;
 #pragma unroll
               for (int q2 = 0; q2 < KQ; ++q2) {
-                const int32_t v = xl_wait(xword(max(irid[u], 0), blk, lane + 64 * q2), ir[u][q2],
-                                          impl[q2], A.xl_epoch, A.xl_abort, ticket,
-                                          (1 << 24) | k);
-                if (impl[q2]) wrow_ptr[wb + 1 + lane + 64 * q2] = (uint16_t)v;
+                const bool stale = impl[q2] && !((irok[u] >> q2) & 1u);
+                const bool any = __ballot(stale) != 0;  // uniform
+                const int32_t v =
+                    any ? xl_wait(xword(max(irid[u], 0), blk, lane + 64 * q2), 0ull, stale,
+                                  A.xl_epoch, A.xl_abort, ticket, (1 << 24) | k)
+                        : 0;
+                const int32_t val = stale ? v : irv[u][q2];
+                if (impl[q2]) wrow_ptr[wb + 1 + lane + 64 * q2] = (uint16_t)val;
               }
             }
             const int32_t rid = __builtin_amdgcn_readlane(mrid, i);
@@ -622,7 +691,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
               if (xinf[j] >= 0) {
                 const int32_t kx = xinf[j] & 0xFFFF;
                 for (int32_t bb = blk + 1; bb <= (xinf[j] >> 16); ++bb)
-                  xl_store(xword(rid, bb, kx), A.xl_epoch, h[j] - negt[j]);
+                  xl_store(xword(rid, bb, kx), A.xl_epoch, h[j] - negt[j], seq);
               }
             }
           }
@@ -652,6 +721,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     mrid = nrid;
     if (q + 2 < nchunks) load_meta(q + 2, nlab, nrf, nslot, nrid);
   }
+  } while (kSeqForm && seq && ++blk < nb);  // blocks of a table swept by one workgroup
 #undef TCOL
 }
 

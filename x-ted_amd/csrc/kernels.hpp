@@ -103,6 +103,7 @@ struct DPArgs {
   int32_t* xl_abort;             // [0] raised when a wait gives up, [1..6] what it waited for
   uint32_t xl_epoch;             // tag of this launch's words (never 0)
   int32_t xl_nbmax;              // workgroups per task in the grid
+  int32_t xl_seq_rows;           // tables this short: one workgroup sweeps all blocks
 };
 // Arena layout of the column-blocked tasks of a batch (after the scatter).
 struct XLLayoutArgs {

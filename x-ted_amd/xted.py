@@ -68,6 +68,7 @@ _SIGNATURES = {
     "xted_set_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_level_streams": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_tail_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "xted_set_xl_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_class_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_int64),
                                           ctypes.POINTER(ctypes.c_int64),
@@ -264,6 +265,9 @@ class Context:
 
     def set_tuning(self, lds_per_wave=10 * 1024, max_cpl=8, min_waves=1, wide_cols=1 << 30):
         _check(lib().xted_set_tuning(self._h, lds_per_wave, max_cpl, min_waves, wide_cols))
+
+    def set_xl_rule(self, seq_rows):
+        _check(lib().xted_set_xl_rule(self._h, seq_rows))
 
     def set_tail_rule(self, tail_pairs):
         _check(lib().xted_set_tail_rule(self._h, tail_pairs))
