@@ -41,15 +41,16 @@ constexpr int kBig = INT_MAX / 4;
 #define DPP_ROW_BCAST31 0x143
 
 // Inclusive prefix minimum over the 64 lanes of a wave (lane order). A lane
-// whose DPP source is out of range keeps its own value as the "old"
-// operand, which is the identity of min(v, v).
+// whose DPP source is out of range (or whose row the mask leaves out) reads
+// INT_MAX, the identity of min; with that identity as the "old" operand the
+// compiler fuses each move into its min (one v_min_i32 with a DPP source).
 __device__ __forceinline__ int wave_incl_min(int v) {
-  v = min(v, __builtin_amdgcn_update_dpp(v, v, DPP_ROW_SHR(1), 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(v, v, DPP_ROW_SHR(2), 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(v, v, DPP_ROW_SHR(4), 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(v, v, DPP_ROW_SHR(8), 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(v, v, DPP_ROW_BCAST15, 0xa, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(v, v, DPP_ROW_BCAST31, 0xc, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(1), 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(2), 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(4), 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(8), 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_BCAST15, 0xa, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_BCAST31, 0xc, 0xf, false));
   return v;
 }
 
@@ -282,6 +283,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // row r-1's total while a slower wave still reads row r's.
   __shared__ int32_t xtot[2][W];
   __shared__ int32_t xsh[kXL ? kMaxImports + 1 : 1];  // import flags, then the ticket
+  __shared__ uint16_t sink[64];  // target of the saved-row stores a lane does not make
 
   int64_t tix = blockIdx.x;
   int32_t blk = 0, ticket = 0;
@@ -380,10 +382,8 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // their stores are masked.
   int32_t bfar[C], labc[C], foff[C], woff[C], Fpm[C], negt[C], tdq[S][C];
   int32_t xinf[kXL ? C : 1];
-  int32_t fpr0[G];  // F[r+1][c+1] of each group's element e = 0
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    fpr0[g] = TCOL(g, 0);  // F[rmax][c+1] = cmax-1-c = t
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int j = g * E + e;
@@ -402,6 +402,9 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       woff[j] = writes ? lrm - lbase : -1;
       Fpm[j] = 2;  // F[rmax][c] + 1 - t = cmax - c + 1 - t
       negt[j] = -t;
+      // Opaque to the optimiser, so that sums with -t stay three-operand
+      // adds (v_add3_u32) instead of becoming an add and a subtract.
+      asm("" : "+v"(negt[j]));
       if constexpr (kXL) {
         // far column oc+1 in an earlier block: an import, by path depth
         const bool imp = !tree && t < ncols && cmax - 2 - oc < tb0;
@@ -546,180 +549,199 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     for (int k = 0; k < PD; ++k) xl_issue(k, k < nrows ? __builtin_amdgcn_readlane(mrid, k) : -1, k);
   }
 
-  for (int32_t q = 0; q < nchunks; ++q) {
-    const int32_t rows_here = min(64, nrows - 64 * q);
-    // Rows go S at a time with the inner loop fully unrolled, so row k of the
-    // table always uses ring slot k % S and every slot index is static: row
-    // r reads slot u and prefetches row r-PD into slot (u+PD) % S, the slot
-    // the previous row has just finished with.
-    for (int32_t i0r = 0; i0r < rows_here; i0r += S) {
+  // F[r+1] at the column just before this wave's block (t = tb0 + w*64*C - 1)
+  // for the row about to start: the boundary row's t + 1 first, then each
+  // row's carry-in plus that t.
+  int32_t left_prev = tb0 + wave * 64 * C;
+
+  // Rows go S at a time in one flat loop with the inner loop fully unrolled,
+  // so row k of the table always uses ring slot k % S and every slot index is
+  // static: row k reads slot u and prefetches row k+PD into slot (u+PD) % S,
+  // the slot the previous row has just finished with. Only the last
+  // iteration leaves the unrolled rows early, and it leaves the loop, so the
+  // prefetches in flight never have to drain at the loop head.
+  for (int32_t k0 = 0; k0 < nrows; k0 += S) {
+    if ((k0 & 63) == 0 && k0 > 0) {  // next 64-row chunk of row metadata
+      mlab = nlab;
+      mrf = nrf;
+      mslot = nslot;
+      mrid = nrid;
+      if ((k0 >> 6) + 1 < nchunks) load_meta((k0 >> 6) + 1, nlab, nrf, nslot, nrid);
+    }
 #pragma unroll
-      for (int u = 0; u < S; ++u) {
-        const int32_t i = i0r + u;
-        if (i >= rows_here) break;
-        const int32_t r = rmax - 1 - (64 * q + i);
-        load_row(max(r - PD, r0), tdq[(u + PD) % S]);
-        if constexpr (kXL) {
-          constexpr int32_t kM = kXLEvery - 1, kHalf = kXLEvery / 2;
-          if ((i & kM) == 0 && (i > 0 || q > 0)) xl_throttle(64 * q + i, true);
-          if ((i & kM) == kHalf && blk > 0 && !seq)
-            tw = xl_load(xl_throttle_word(64 * q + i + kHalf));
-          const int32_t ia = i + PD;
-          const int32_t rid = ia < 64 ? __builtin_amdgcn_readlane(mrid, ia)
-                                      : __builtin_amdgcn_readlane(nrid, ia - 64);
-          xl_issue(64 * q + ia, rid, (u + PD) % S);
+    for (int u = 0; u < S; ++u) {
+      const int32_t k = k0 + u;  // rows done so far
+      if (k >= nrows) break;
+      const int32_t i = k & 63;  // index in the metadata chunk
+      const int32_t r = rmax - 1 - k;
+      load_row(max(r - PD, r0), tdq[(u + PD) % S]);
+      if constexpr (kXL) {
+        constexpr int32_t kM = kXLEvery - 1, kHalf = kXLEvery / 2;
+        if ((k & kM) == 0 && k > 0) xl_throttle(k, true);
+        if ((k & kM) == kHalf && blk > 0 && !seq) tw = xl_load(xl_throttle_word(k + kHalf));
+        const int32_t ia = i + PD;
+        const int32_t rid = ia < 64 ? __builtin_amdgcn_readlane(mrid, ia)
+                                    : __builtin_amdgcn_readlane(nrid, ia - 64);
+        xl_issue(k + PD, rid, (u + PD) % S);
+      }
+
+      const int32_t labr = __builtin_amdgcn_readlane(mlab, i);
+      const int32_t Rf = __builtin_amdgcn_readlane(mrf, i);
+      const int32_t sl = __builtin_amdgcn_readlane(mslot, i);
+      const bool tree_row = (Rf == rmax);
+      const uint16_t* frow = slots + (sl & 0xFFFF) * rowlen;
+      const bool wrow = sl < 0;
+      uint16_t* wrow_ptr = slots + ((sl >> 16) & 0x7FFF) * rowlen;
+
+      // h = a[t] - t per element, local prefix per group, chained wave
+      // scans. Tree rows (orl_R(r) == orl_R(r0)) and the other rows take
+      // separate code paths on the uniform flag.
+      int32_t h[C], pre[G];
+      int32_t carry_w = kBig;  // min over this wave's earlier groups
+      auto row_terms = [&](auto tree_tag) {
+        constexpr bool kTree = decltype(tree_tag)::value;
+        int32_t sv[C];
+        if constexpr (!kTree) {
+#pragma unroll
+          for (int j = 0; j < C; ++j) sv[j] = frow[foff[j]];
         }
-
-        const int32_t labr = __builtin_amdgcn_readlane(mlab, i);
-        const int32_t Rf = __builtin_amdgcn_readlane(mrf, i);
-        const int32_t sl = __builtin_amdgcn_readlane(mslot, i);
-        const bool tree_row = (Rf == rmax);
-        const uint16_t* frow = slots + (sl & 0xFFFF) * rowlen;
-        const bool wrow = sl < 0;
-        uint16_t* wrow_ptr = slots + ((sl >> 16) & 0x7FFF) * rowlen;
-
-        // h = a[t] - t per element, local prefix per group, chained wave
-        // scans. Tree rows (orl_R(r) == orl_R(r0)) and the other rows take
-        // separate code paths on the uniform flag.
-        int32_t h[C], pre[G];
-        int32_t carry_w = kBig;  // min over this wave's earlier groups
-        auto row_terms = [&](auto tree_tag) {
-          constexpr bool kTree = decltype(tree_tag)::value;
-          int32_t sv[C];
-          if constexpr (!kTree) {
-#pragma unroll
-            for (int j = 0; j < C; ++j) sv[j] = frow[foff[j]];
-          }
+        // Tree rows: F[r+1][c+1] of each group's element e = 0, the previous
+        // row's value one column to the left in t order (F = Fpm - 1 + t).
+        int32_t fpr0[G];
+        if constexpr (kTree) {
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-              const int j = g * E + e;
-              int32_t dm;  // diagonal-or-far term minus t
-              if constexpr (kTree) {
-                // diagonal F[r+1][c+1] + rename on a tree column, else the
-                // boundary-row far value. F[r+1][c+1] - t = Fpm[j-1] - 2.
-                const int32_t fprm = e ? Fpm[j - 1] - 2 : fpr0[g] + negt[j];
-                dm = bfar[j] < 0 ? fprm + (int32_t)(labr != labc[j]) : bfar[j] + tdq[u][j] + negt[j];
-              } else {
-                dm = sv[j] + tdq[u][j] + negt[j];
-              }
-              h[j] = min(dm, Fpm[j]);
-              if (e) h[j] = min(h[j], h[j - 1]);
-            }
-            const int32_t incl = wave_incl_min(h[g * E + E - 1]);
-            pre[g] = min(carry_w, wave_shr1(incl, kBig));
-            carry_w = min(carry_w, __builtin_amdgcn_readlane(incl, 63));
-          }
-        };
-        if (tree_row) row_terms(std::true_type{});
-        else row_terms(std::false_type{});
-
-        // Carry into this wave: the boundary F[r][cmax] - (-1), or for a
-        // later column block the carry of the blocks before it, then the
-        // totals of this workgroup's earlier waves.
-        int32_t carry = (rmax - r) + 1;
-        if constexpr (kXL) {
-          if (blk > 0) {
-            const int32_t k = 64 * q + i;
-            carry = (crok >> u) & 1u ? crv[u]
-                                     : xl_wait(xw + cin_off + k, 0ull, true, A.xl_epoch,
-                                               A.xl_abort, ticket, k);
+            const int jl = g * E + E - 1;  // the group's last column
+            const int32_t fill =
+                g ? __builtin_amdgcn_readlane(Fpm[jl - E] - 1 - negt[jl - E], 63) : left_prev;
+            fpr0[g] = wave_shr1(Fpm[jl] - 1 - negt[jl], fill);
           }
         }
-        if constexpr (W > 1) {
-          int32_t* xt = xtot[r & 1];
-          if (lane == 0) xt[wave] = carry_w;
-          __syncthreads();
-          for (int w = 0; w < wave; ++w) carry = min(carry, xt[w]);
-        }
-        if constexpr (kXL) {
-          // The next block's carry: this block's last wave, carry-in
-          // combined with its own total.
-          if (blk + 1 < nb && wave == W - 1 && lane == 0)
-            xl_store(xw + cout_off + (64 * q + i), A.xl_epoch, min(carry, carry_w), seq);
-        }
-        // h becomes G - t; Fpm = G + 1 - t = h + 1.
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-          const int32_t cg = min(carry, pre[g]);
 #pragma unroll
           for (int e = 0; e < E; ++e) {
             const int j = g * E + e;
-            h[j] = min(cg, h[j]);
-            Fpm[j] = h[j] + 1;
+            int32_t dm;  // diagonal-or-far term minus t
+            if constexpr (kTree) {
+              // diagonal F[r+1][c+1] + rename on a tree column, else the
+              // boundary-row far value. F[r+1][c+1] - t = Fpm[j-1] - 2.
+              // Blended with a mask rather than a branch per element.
+              const int32_t fprm = e ? Fpm[j - 1] - 2 : fpr0[g] + negt[j];
+              const int32_t diag = fprm + (int32_t)(labr != labc[j]);
+              const int32_t far = bfar[j] + tdq[u][j] + negt[j];
+              const int32_t m = bfar[j] >> 31;  // all ones on a tree column
+              dm = (diag & m) | (far & ~m);
+            } else {
+              dm = sv[j] + tdq[u][j] + negt[j];
+            }
+            h[j] = min(dm, Fpm[j]);
+            if (e) h[j] = min(h[j], h[j - 1]);
           }
+          const int32_t incl = wave_incl_min(h[g * E + E - 1]);
+          // (values stay far below kBig, so group 0 needs no min with it)
+          pre[g] = g ? min(carry_w, wave_shr1(incl, kBig)) : wave_shr1(incl, kBig);
+          carry_w = g ? min(carry_w, __builtin_amdgcn_readlane(incl, 63))
+                      : __builtin_amdgcn_readlane(incl, 63);
         }
+      };
+      if (tree_row) row_terms(std::true_type{});
+      else row_terms(std::false_type{});
 
-        // Tree cells -> TD / TD^T (tree rows only); saved rows -> LDS slots.
-        if (tree_row) {
-#pragma unroll
-          for (int j = 0; j < C; ++j) {
-            const int32_t t = TCOL(j / E, j % E);
-            if (t < ncols && bfar[j] < 0) {
-              const int32_t c = cmax - 1 - t;
-              TDo[(int64_t)r * ldo + c] = h[j] + t;
-              TDt[(int64_t)c * ldt + r] = h[j] + t;
-            }
-          }
-        }
-        if (wrow) {
-#pragma unroll
-          for (int j = 0; j < C; ++j)
-            if (woff[j] >= 0) wrow_ptr[woff[j]] = (uint16_t)(h[j] - negt[j]);
-          if (threadIdx.x == 0) wrow_ptr[wb] = (uint16_t)(rmax - r);
-          if constexpr (kXL) {
-            // Imported far values of this saved row; far values later
-            // blocks import from this one.
-            if (wave == 0) {
-              const int32_t k = 64 * q + i;
-#pragma unroll
-              for (int q2 = 0; q2 < KQ; ++q2) {
-                const bool stale = impl[q2] && !((irok[u] >> q2) & 1u);
-                const bool any = __ballot(stale) != 0;  // uniform
-                const int32_t v =
-                    any ? xl_wait(xword(max(irid[u], 0), blk, lane + 64 * q2), 0ull, stale,
-                                  A.xl_epoch, A.xl_abort, ticket, (1 << 24) | k)
-                        : 0;
-                const int32_t val = stale ? v : irv[u][q2];
-                if (impl[q2]) wrow_ptr[wb + 1 + lane + 64 * q2] = (uint16_t)val;
-              }
-            }
-            const int32_t rid = __builtin_amdgcn_readlane(mrid, i);
-#pragma unroll
-            for (int j = 0; j < C; ++j) {
-              if (xinf[j] >= 0) {
-                const int32_t kx = xinf[j] & 0xFFFF;
-                for (int32_t bb = blk + 1; bb <= (xinf[j] >> 16); ++bb)
-                  xl_store(xword(rid, bb, kx), A.xl_epoch, h[j] - negt[j], seq);
-              }
-            }
-          }
-          // The next row may read this slot row at columns of other waves.
-          // (Every other LDS reuse is ordered by the barrier above: a slot
-          // row is only rewritten after the rows that read it.)
-          if constexpr (W > 1) __syncthreads();
-        }
-
-        // F[r][c] at the column just before this wave's block (t = tb0 +
-        // w*64*C - 1). The previous wave's last element ends with h =
-        // min(its carry-in, its own total) = this wave's carry-in, and F =
-        // h + t; for wave 0 of block 0 this gives F[r][cmax] = rmax - r.
-        const int32_t left_in = carry + (tb0 + wave * 64 * C - 1);
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          const int jl = g * E + E - 1;  // the group's last column
-          const int32_t fill =
-              g ? __builtin_amdgcn_readlane(h[jl - E] - negt[jl - E], 63) : left_in;
-          fpr0[g] = wave_shr1(h[jl] - negt[jl], fill);
+      // Carry into this wave: the boundary F[r][cmax] - (-1), or for a
+      // later column block the carry of the blocks before it, then the
+      // totals of this workgroup's earlier waves.
+      int32_t carry = (rmax - r) + 1;
+      if constexpr (kXL) {
+        if (blk > 0) {
+          carry = (crok >> u) & 1u ? crv[u]
+                                   : xl_wait(xw + cin_off + k, 0ull, true, A.xl_epoch,
+                                             A.xl_abort, ticket, k);
         }
       }
+      if constexpr (W > 1) {
+        int32_t* xt = xtot[r & 1];
+        if (lane == 0) xt[wave] = carry_w;
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < W - 1; ++w) carry = min(carry, w < wave ? xt[w] : kBig);
+      }
+      if constexpr (kXL) {
+        // The next block's carry: this block's last wave, carry-in
+        // combined with its own total.
+        if (blk + 1 < nb && wave == W - 1 && lane == 0)
+          xl_store(xw + cout_off + k, A.xl_epoch, min(carry, carry_w), seq);
+      }
+      // F[r][c] at the column just before this wave's block: the previous
+      // wave's last element ends with h = min(its carry-in, its own total) =
+      // this wave's carry-in, and F = h + t; for wave 0 of block 0 this gives
+      // F[r][cmax] = rmax - r.
+      left_prev = carry + (tb0 + wave * 64 * C - 1);
+      // h becomes G - t; Fpm = G + 1 - t = h + 1.
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int32_t cg = min(carry, pre[g]);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int j = g * E + e;
+          h[j] = min(cg, h[j]);
+          Fpm[j] = h[j] + 1;
+        }
+      }
+
+      // Tree cells -> TD / TD^T (tree rows only); saved rows -> LDS slots.
+      if (tree_row) {
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+          const int32_t t = TCOL(j / E, j % E);
+          if (t < ncols && bfar[j] < 0) {
+            const int32_t c = cmax - 1 - t;
+            TDo[(int64_t)r * ldo + c] = h[j] + t;
+            TDt[(int64_t)c * ldt + r] = h[j] + t;
+          }
+        }
+      }
+      if (wrow) {
+        // Columns this lane does not save write to a sink entry of their
+        // own instead of branching per element.
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+          uint16_t* dst = woff[j] >= 0 ? wrow_ptr + woff[j] : sink + lane;
+          *dst = (uint16_t)(h[j] - negt[j]);
+        }
+        if (threadIdx.x == 0) wrow_ptr[wb] = (uint16_t)(rmax - r);
+        if constexpr (kXL) {
+          // Imported far values of this saved row; far values later
+          // blocks import from this one.
+          if (wave == 0) {
+#pragma unroll
+            for (int q2 = 0; q2 < KQ; ++q2) {
+              const bool stale = impl[q2] && !((irok[u] >> q2) & 1u);
+              const bool any = __ballot(stale) != 0;  // uniform
+              const int32_t v =
+                  any ? xl_wait(xword(max(irid[u], 0), blk, lane + 64 * q2), 0ull, stale,
+                                A.xl_epoch, A.xl_abort, ticket, (1 << 24) | k)
+                      : 0;
+              const int32_t val = stale ? v : irv[u][q2];
+              if (impl[q2]) wrow_ptr[wb + 1 + lane + 64 * q2] = (uint16_t)val;
+            }
+          }
+          const int32_t rid = __builtin_amdgcn_readlane(mrid, i);
+#pragma unroll
+          for (int j = 0; j < C; ++j) {
+            if (xinf[j] >= 0) {
+              const int32_t kx = xinf[j] & 0xFFFF;
+              for (int32_t bb = blk + 1; bb <= (xinf[j] >> 16); ++bb)
+                xl_store(xword(rid, bb, kx), A.xl_epoch, h[j] - negt[j], seq);
+            }
+          }
+        }
+        // The next row may read this slot row at columns of other waves.
+        // (Every other LDS reuse is ordered by the barrier above: a slot
+        // row is only rewritten after the rows that read it.)
+        if constexpr (W > 1) __syncthreads();
+      }
     }
-    mlab = nlab;
-    mrf = nrf;
-    mslot = nslot;
-    mrid = nrid;
-    if (q + 2 < nchunks) load_meta(q + 2, nlab, nrf, nslot, nrid);
   }
   } while (kSeqForm && seq && ++blk < nb);  // blocks of a table swept by one workgroup
 #undef TCOL
