@@ -351,9 +351,10 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
     g.pm[i] = PairMeta{x, y, td_total + kTdPad, td_total + 2 * kTdPad + mn};
     td_total += 2 * mn + 3 * kTdPad;
     // With the leaf kernel on, the plan covers only non-leaf keyroot pairs;
-    // the leaf kernel writes one line per leaf keyroot of either tree.
+    // the leaf kernel writes one line per leaf keyroot of either tree, in
+    // tiles of up to 64 lines of one side.
     const int32_t ka = lane_enable ? a.K - a.K1 : a.K, kb = lane_enable ? b.K - b.K1 : b.K;
-    g.lines[i + 1] = g.lines[i] + (lane_enable ? a.K1 + b.K1 : 0);
+    g.lines[i + 1] = g.lines[i] + (lane_enable ? (a.n + 63) / 64 + (b.n + 63) / 64 : 0);
     if (lane_enable) {
       g.leaf_tables += (int64_t)a.K1 * b.K + (int64_t)(a.K - a.K1) * b.K1;
       g.leaf_cells += (int64_t)a.K1 * info_[y].ksum + (int64_t)b.K1 * (info_[x].ksum - a.K1);
@@ -493,16 +494,16 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<std::pair<int, 
     int cls, bucket;
     unsigned long long off, cnt, cells;
   };
-  std::vector<Launch> lv;
+  std::vector<Launch> lv, leaf;
   for (int gi = 0; gi < ng; ++gi) {
     Group& g = *gs[gi];
     const int L = g.next_level;
     if (L >= g.nlevels) continue;
-    // The leaf lines read nothing and level-0 tables (two chains) read no TD,
-    // so the leaf kernel runs beside level 0.
+    // The leaf lines read nothing; the leaf kernel also fills cells of later
+    // tables with placeholders, so it runs before the group's first level.
     if (L == 0 && g.lines[g.npairs] > 0)
-      lv.push_back({&g, kClassLane, -1, 0, (unsigned long long)g.lines[g.npairs],
-                    (unsigned long long)g.leaf_cells});
+      leaf.push_back({&g, kClassLane, -1, 0, (unsigned long long)g.lines[g.npairs],
+                      (unsigned long long)g.leaf_cells});
     unsigned long long off = g.level_off;
     for (int c = 0; c < kNumClasses; ++c) {
       const size_t b = (size_t)L * kNumClasses + c;
@@ -515,26 +516,32 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<std::pair<int, 
   std::stable_sort(lv.begin(), lv.end(),
                    [](const Launch& x, const Launch& y) { return x.cells > y.cells; });
   if (prof) {
-    while (prof_ev_.size() < 2 * (prof->size() + lv.size())) {
+    while (prof_ev_.size() < 2 * (prof->size() + lv.size() + leaf.size())) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
       prof_ev_.push_back(e);
     }
   }
   hipStream_t sl[kMaxLevelStreams];
-  const int ns = std::min<int>((int)lv.size(), lane_streams(lane, sl));
-  if (ns > 1) HIP_TRY(hipEventRecord(fork_ev_, sl[0]));
-  for (int k = 1; k < ns; ++k) HIP_TRY(hipStreamWaitEvent(sl[k], fork_ev_, 0));
+  const int ns = std::max(1, std::min<int>((int)lv.size(), lane_streams(lane, sl)));
+  // Leaf launches go first on the main stream; the fork follows them, so
+  // every table of the level starts after them.
+  const size_t nleaf = leaf.size();
+  lv.insert(lv.begin(), leaf.begin(), leaf.end());
   for (size_t i = 0; i < lv.size(); ++i) {
+    if (i == nleaf) {
+      if (ns > 1) HIP_TRY(hipEventRecord(fork_ev_, sl[0]));
+      for (int k = 1; k < ns; ++k) HIP_TRY(hipStreamWaitEvent(sl[k], fork_ev_, 0));
+    }
     const Launch& w = lv[i];
     Group& g = *w.g;
-    hipStream_t s = sl[i % ns];
+    hipStream_t s = i < nleaf ? sl[0] : sl[(i - nleaf) % ns];
     if (prof) HIP_TRY(hipEventRecord(prof_ev_[2 * prof->size()], s));
     if (w.cls == kClassLane) {
       LeafArgs la;
       la.pool = view();
       la.pairs = g.pairs.as<PairMeta>();
-      la.line_start = g.lines_d.as<int64_t>();
+      la.tile_start = g.lines_d.as<int64_t>();
       la.npairs = g.npairs;
       la.td = g.td.as<int32_t>();
       launch_leaf(la, (int64_t)w.cnt, s);

@@ -61,7 +61,7 @@ struct Group {
   int64_t nxl = 0, leaf_tables = 0, leaf_cells = 0, td_bytes = 0;
   std::vector<int32_t> xs, ys;
   std::vector<PairMeta> pm;
-  std::vector<int64_t> lines, xl_first;
+  std::vector<int64_t> lines, xl_first;  // lines: prefix of leaf-kernel tiles per pair
   std::vector<unsigned long long> h_counts, h_cells;
   std::vector<int32_t> h_maxlds;
   std::vector<uint8_t> h_tail;

@@ -60,58 +60,116 @@ __device__ __forceinline__ int wave_shr1(int v, int fill) {
 }
 
 // ---------------------------------------------------------------------------
-// Leaf-keyroot lines in closed form. Line q of pair p is a leaf keyroot v of
-// one tree; the wave walks the other tree T right to left, 64 positions per
-// step, lane l at position c = hi - l. nxt(c), the first position >= c whose
-// label is label(v), is a prefix minimum over lanes plus the minimum carried
-// from the positions above, and TD = (orl(c) - c + 1) - [nxt(c) <= orl(c)].
-// The line is contiguous in one of TD / TD^T (coalesced) and strided in the
-// other.
+// Leaf-keyroot lines in closed form. A line is a leaf keyroot v of one tree R
+// of a pair; over the other tree T it is TD(v, c) = (orl(c) - c + 1) -
+// [nxt(c) <= orl(c)], nxt(c) being the first position >= c whose label is
+// label(v). The line is a row of TD (v of x) or of TD^T (v of y) and a column
+// of the other layout.
+//
+// One workgroup of 4 waves takes the leaf keyroots among 64 consecutive
+// nodes of one side (a tile) and walks T right to left, 64 positions per
+// step, lane l at position c = top - l. For each of its lines a wave forms
+// nxt as a prefix minimum over lanes plus the minimum carried from the
+// positions above, writes the 64 values as one contiguous row segment and
+// keeps them in an LDS tile; then the workgroup writes the tile transposed,
+// one position per wave instruction with the tile's 64 nodes across the
+// lanes: whole 256-byte runs of the column layout. The entries of nodes that
+// are not leaf keyroots carry no value yet; they belong to tables of two
+// non-leaf keyroots, which run after this kernel and overwrite them, and no
+// table reads a cell before the level that writes it. Positions that are
+// leaf keyroots of T are left out: their whole row is a line of the other
+// side, written by its tiles.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void leaf_kernel(LeafArgs A, int64_t nlines) {
-  const int64_t line = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (line >= nlines) return;  // uniform per wave
+constexpr int kLeafTile = 64;
+__global__ __launch_bounds__(256) void leaf_kernel(LeafArgs A, int64_t ntiles) {
+  __shared__ int32_t tile[kLeafTile][kLeafTile + 1];
+  __shared__ int32_t lines[kLeafTile];
+  const int64_t tix = blockIdx.x;
+  if (tix >= ntiles) return;
+  const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   int lo = 0, hi = A.npairs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (A.line_start[mid] <= line) lo = mid;
+    if (A.tile_start[mid] <= tix) lo = mid;
     else hi = mid - 1;
   }
   const PairMeta pm = A.pairs[lo];
   const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
-  const int32_t q = (int32_t)(line - A.line_start[lo]);
-  const bool of_x = q < tx.K1;  // leaf keyroot of x: the line is TD[v][.]
+  const int32_t q = (int32_t)(tix - A.tile_start[lo]);
+  const int32_t qx = (tx.n + kLeafTile - 1) / kLeafTile;
+  const bool of_x = q < qx;  // nodes of x: the lines are rows of TD
   const TreeMeta& R = of_x ? tx : ty;
   const TreeMeta& T = of_x ? ty : tx;
-  const int32_t v = A.pool.skr[R.node_off + (of_x ? q : q - tx.K1)];
-  const int32_t lam = A.pool.label[R.node_off + v];
+  const int32_t v0 = (of_x ? q : q - qx) * kLeafTile;
+  // This lane's node of the tile, and whether it is a leaf keyroot.
+  const int32_t vl = v0 + lane;
+  const int32_t vc = min(vl, R.n - 1);
+  const bool is_line = vl < R.n && A.pool.orl[R.node_off + vc] == vc &&
+                       A.pool.kdn[R.node_off + vc] >= 0;
+  const unsigned long long lmask = __ballot(is_line);
+  const int nl = __popcll(lmask);
+  if (nl == 0) return;  // uniform over the workgroup
+  if (wave == 0 && is_line) lines[__popcll(lmask & ((1ull << lane) - 1))] = lane;
+  const int32_t laml = A.pool.label[R.node_off + vc];
   const int32_t* __restrict__ labT = A.pool.label + T.node_off;
   const int32_t* __restrict__ orlT = A.pool.orl + T.node_off;
+  const int32_t* __restrict__ kdnT = A.pool.kdn + T.node_off;
   // of_x: TD[v][c] = td[td_off + v*ny + c], TD^T[c][v] = td[tdt_off + c*nx + v];
   // else: TD^T[v][c] = td[tdt_off + v*nx + c], TD[c][v] = td[td_off + c*ny + v].
-  int32_t* pc = A.td + (of_x ? pm.td_off + (int64_t)v * ty.n : pm.tdt_off + (int64_t)v * tx.n);
-  int32_t* ps = A.td + (of_x ? pm.tdt_off : pm.td_off) + v;
-  const int64_t lds = of_x ? tx.n : ty.n;
-  int32_t carry = INT_MAX;
-  for (int32_t top = T.n - 1; top >= 0; top -= 64) {
+  int32_t* const rows = A.td + (of_x ? pm.td_off : pm.tdt_off);  // [v][c], stride T.n
+  int32_t* const cols = A.td + (of_x ? pm.tdt_off : pm.td_off);  // [c][v], stride R.n
+  const bool col_ok = vl < R.n;
+  __syncthreads();
+  constexpr int kPer = kLeafTile / 4;  // lines (and positions) per wave
+  int32_t carry[kPer];
+#pragma unroll
+  for (int m = 0; m < kPer; ++m) carry[m] = INT_MAX;
+  // Positions of the next step are read one step ahead.
+  int32_t top = T.n - 1;
+  int32_t cc = max(top - lane, 0);
+  int32_t nlab = labT[cc], norl = orlT[cc], nkdn = kdnT[cc];
+  for (; top >= 0; top -= 64) {
     const int32_t c = top - lane;
-    const int32_t cc = max(c, 0);
-    const int32_t lab = labT[cc], oc = orlT[cc];
-    const int32_t incl = wave_incl_min(c >= 0 && lab == lam ? c : INT_MAX);
-    const int32_t nxt = min(carry, incl);
-    carry = min(carry, __builtin_amdgcn_readlane(incl, 63));
-    const int32_t d = (oc - c + 1) - (int32_t)(nxt <= oc);
-    if (c >= 0) {
-      pc[c] = d;
-      ps[(int64_t)c * lds] = d;
+    const int32_t lab = nlab, oc = norl;
+    const bool c_line = oc == c && nkdn >= 0;  // c is a leaf keyroot of T
+    if (top >= 64) {
+      cc = max(top - 64 - lane, 0);
+      nlab = labT[cc];
+      norl = orlT[cc];
+      nkdn = kdnT[cc];
     }
+    const int32_t size = oc - c + 1;
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) {
+      const int k = wave + 4 * m;  // lines interleaved over the waves
+      if (k < nl) {                // uniform
+        const int32_t off = lines[k];
+        const int32_t lam = __builtin_amdgcn_readlane(laml, off);
+        const int32_t incl = wave_incl_min(c >= 0 && lab == lam ? c : INT_MAX);
+        const int32_t nxt = min(carry[m], incl);
+        carry[m] = min(carry[m], __builtin_amdgcn_readlane(incl, 63));
+        const int32_t d = size - (int32_t)(nxt <= oc);
+        if (c >= 0) rows[(int64_t)(v0 + off) * T.n + c] = d;
+        tile[off][lane] = d;
+      }
+    }
+    const unsigned long long skip = __ballot(c < 0 || c_line);
+    __syncthreads();
+    // Transposed: position p = wave + 4m of this step, node v0 + lane.
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) {
+      const int p = wave + 4 * m;
+      if (!((skip >> p) & 1ull) && col_ok)
+        cols[(int64_t)(top - p) * R.n + vl] = tile[lane][p];
+    }
+    __syncthreads();
   }
 }
 
-void launch_leaf(const LeafArgs& a, int64_t nlines, hipStream_t s) {
-  if (nlines <= 0) return;
-  hipLaunchKernelGGL(leaf_kernel, dim3((unsigned)((nlines + 3) / 4)), dim3(256), 0, s, a, nlines);
+void launch_leaf(const LeafArgs& a, int64_t ntiles, hipStream_t s) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(leaf_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, a, ntiles);
 }
 
 // ---------------------------------------------------------------------------

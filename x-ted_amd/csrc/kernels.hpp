@@ -78,12 +78,15 @@ void launch_gather_dist(const PairMeta* pairs, const int32_t* td, int32_t npairs
 struct LeafArgs {
   PoolView pool;
   const PairMeta* pairs;
-  const int64_t* line_start;  // npairs+1: prefix over pairs of K1_x + K1_y
+  // npairs+1: prefix over pairs of ceil(n_x / 64) + ceil(n_y / 64)
+  const int64_t* tile_start;
   int32_t npairs;
   int32_t* td;
 };
-// One wave per line (a leaf keyroot of x or of y of one pair).
-void launch_leaf(const LeafArgs& a, int64_t nlines, hipStream_t s);
+// One workgroup per tile of 64 consecutive nodes of one side of a pair (its
+// leaf keyroots are the lines). Writes cells of other tables' nodes as well
+// (see forest.hip), so it must finish before any table of the group runs.
+void launch_leaf(const LeafArgs& a, int64_t ntiles, hipStream_t s);
 
 // Ints of padding the TD arena keeps before and after every table, so the
 // row-scan kernels can issue whole-group loads without bounds branches.
