@@ -177,7 +177,7 @@ void launch_leaf(const LeafArgs& a, int64_t ntiles, hipStream_t s) {
 // lane. Columns are counted from jmax: t = jmax-1-j. Wave w owns the block
 // t in [w*64*C, (w+1)*64*C). Inside it, lanes own E contiguous columns per
 // group and the G = C/E groups are interleaved: lane l's element (g, e) is
-//   t = w*64*C + g*64*E + l*E + e,   E = min(C, 4).
+//   t = w*64*C + g*64*E + l*E + e,   E = min(C, 8).
 // So one group of a TD row is a 64*E-int contiguous span that the wave
 // reads with one coalesced E-int load per lane, and the row scan is G
 // chained wave scans over groups.
@@ -318,14 +318,14 @@ __device__ __forceinline__ int64_t xl_words(const XLGeom& g) {
 // jmax: t = jmax-1-j. Wave w owns the block t in [tb0 + w*64*C,
 // tb0 + (w+1)*64*C). Inside it, lanes own E contiguous columns per group and
 // the G = C/E groups are interleaved: lane l's element (g, e) is
-//   t = tb0 + w*64*C + g*64*E + l*E + e,   E = min(C, 4).
+//   t = tb0 + w*64*C + g*64*E + l*E + e,   E = min(C, 8).
 // So one group of a TD row is a 64*E-int contiguous span that the wave
 // reads with one coalesced E-int load per lane, and the row scan is G
 // chained wave scans over groups.
 // ---------------------------------------------------------------------------
 template <int W, int C, bool kXL>
 __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
-  constexpr int E = C <= 4 ? C : 4;  // contiguous columns per group
+  constexpr int E = C <= 8 ? C : 8;  // contiguous columns per group
   constexpr int G = C / E;           // groups per lane
   // TD ring slots (a power of two dividing 64): up to 32 registers of ring,
   // so narrow lanes prefetch more rows ahead.
@@ -496,7 +496,18 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     for (int g = 0; g < G; ++g) {
       const int32_t tb = TCOL(g, 0);
       const int32_t off = tb < ncols ? cmax - E - tb : c0;
-      if constexpr (E == 4) {
+      if constexpr (E == 8) {
+        const int4u lo = *reinterpret_cast<const int4u*>(row + off);
+        const int4u hi = *reinterpret_cast<const int4u*>(row + off + 4);
+        out[g * 8 + 0] = hi.w;
+        out[g * 8 + 1] = hi.z;
+        out[g * 8 + 2] = hi.y;
+        out[g * 8 + 3] = hi.x;
+        out[g * 8 + 4] = lo.w;
+        out[g * 8 + 5] = lo.z;
+        out[g * 8 + 6] = lo.y;
+        out[g * 8 + 7] = lo.x;
+      } else if constexpr (E == 4) {
         const int4u v = *reinterpret_cast<const int4u*>(row + off);
         out[g * 4 + 0] = v.w;
         out[g * 4 + 1] = v.z;

@@ -90,7 +90,7 @@ void launch_leaf(const LeafArgs& a, int64_t ntiles, hipStream_t s);
 
 // Ints of padding the TD arena keeps before and after every table, so the
 // row-scan kernels can issue whole-group loads without bounds branches.
-constexpr int kTdPad = 4;
+constexpr int kTdPad = 8;
 
 // Forest DP.
 struct DPArgs {
