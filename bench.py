@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the granularity rule first and use the fastest")
     ap.add_argument("--no-xl", action="store_true", help="skip the 20,000-node pair")
+    ap.add_argument("--batch-groups", type=int, default=0,
+                    help="groups a batch is cut into (0: the library default)")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="also run the serving leg (Poisson arrivals) for this long")
     ap.add_argument("--serve-rate", type=float, default=50.0)
@@ -259,6 +261,8 @@ def main():
     else:
         rule = autotune.load()
         autotune.apply(ctx, rule)
+    if args.batch_groups:
+        ctx.set_batch_groups(args.batch_groups)
     log(f"granularity rule: {rule}")
 
     dist = None

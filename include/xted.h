@@ -128,6 +128,12 @@ int xted_set_xl_rule(xted_context* ctx, int32_t seq_rows);
  * launch before the next (X-TED_GPU/parallel_X_TED.cu:433-438). Results do
  * not depend on it. */
 int xted_set_level_streams(xted_context* ctx, int32_t n);
+/* A batch of at least split_min pairs is cut into groups (1 to 4; pair i in
+ * group i mod groups), each planned and run on its own stream: one group's
+ * level barriers and last tables overlap the others' work, and a group's
+ * plan overlaps the levels of the groups before it. No counterpart in the
+ * reference, which runs one pair at a time. Results do not depend on it. */
+int xted_set_batch_groups(xted_context* ctx, int32_t groups, int32_t split_min);
 
 /* ---- per-class device timing ---------------------------------------------------
  * With profiling on, xted_ted / xted_batch bracket every forest-DP launch

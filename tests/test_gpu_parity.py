@@ -179,6 +179,26 @@ def _mixed_batch(ctx, npairs, seed):
     return trees, ids[0::2].copy(), ids[1::2].copy(), shapes
 
 
+def test_batch_groups_do_not_change_results(gpu_ctx):
+    """A batch cut into 1, 2, 3 or 4 independently scheduled groups (split
+    from 2 pairs up): tables identical, distances in the caller's order and
+    equal to the oracle's."""
+    trees, xs, ys, shapes = _mixed_batch(gpu_ctx, 37, 2)
+    out = {}
+    try:
+        for n in (1, 2, 3, 4):
+            gpu_ctx.set_batch_groups(n, 2)
+            out[n] = gpu_ctx.batch(xs, ys, td_shapes=shapes)
+    finally:
+        gpu_ctx.set_batch_groups(1, 256)
+    for n in (2, 3, 4):
+        assert np.array_equal(out[n][0], out[1][0])
+        for a, b in zip(out[n][1], out[1][1]):
+            assert np.array_equal(a, b)
+    for k in range(0, 37, 4):
+        assert out[4][0][k] == oracle.ted(trees[2 * k], trees[2 * k + 1], threads=4)[0]
+
+
 def test_level_streams_do_not_change_results(gpu_ctx):
     """Launches of one level spread over 1, 2 or 4 streams: tables identical,
     distances equal to the oracle's."""

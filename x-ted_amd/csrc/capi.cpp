@@ -240,6 +240,17 @@ int xted_set_level_streams(xted_context* ctx, int32_t n) {
   return OK;
 }
 
+int xted_set_batch_groups(xted_context* ctx, int32_t groups, int32_t split_min) {
+  if (!ctx || groups < 1 || groups > kMaxLevelStreams || split_min < 1) {
+    set_error("xted_set_batch_groups: groups must be 1 to " + std::to_string(kMaxLevelStreams) +
+              " and split_min positive");
+    return ERR_INVALID_ARG;
+  }
+  ctx->ctx.batch_groups = groups;
+  ctx->ctx.batch_split_min = split_min;
+  return OK;
+}
+
 int xted_set_profile(xted_context* ctx, int32_t on) {
   if (!ctx) return ERR_INVALID_ARG;
   ctx->ctx.profile = on ? 1 : 0;

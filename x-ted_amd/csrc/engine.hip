@@ -10,6 +10,7 @@
 //      boundaries order the levels, so no host sync between levels
 //   5. distances gathered on the device and copied back (optionally TDs)
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstring>
 
@@ -77,7 +78,7 @@ Context::~Context() {
     (void)hipSetDevice(device_);
     (void)hipDeviceSynchronize();
   }
-  batch_.release();
+  for (Group& g : batch_) g.release();
   for (DevBuf* b : {&parent_, &label_, &orl_, &kdep_, &kdn_, &lrank_, &kr_, &krkd_, &krsize_,
                     &krleaf_, &trees_, &prep_scratch_, &prep_off_, &skr_, &skd_, &ssize_, &sleaf_,
                     &spre_, &kdoff_, &rtop_})
@@ -291,8 +292,10 @@ void Group::release() {
   if (h_dist_pin) (void)hipHostFree(h_dist_pin);
   h_dist_pin = nullptr;
   pin_cap = 0;
-  if (done_ev) (void)hipEventDestroy(done_ev);
-  done_ev = nullptr;
+  for (hipEvent_t* e : {&done_ev, &ev_plan0, &ev_plan1, &ev_end}) {
+    if (*e) (void)hipEventDestroy(*e);
+    *e = nullptr;
+  }
   for (DevBuf* b : {&pairs, &blk, &counts, &cells, &cursor, &maxlds, &tickets, &abort, &tasks, &td,
                     &dist, &lines_d, &tail_lv, &xl_off, &xl_total, &xl_words})
     b->release();
@@ -373,7 +376,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   g.h_tail.assign(nlevels, 0);
   for (int L = nlevels - 1, run = 0; L >= 0; --L) {
     run += active[L];
-    g.h_tail[L] = (tail_pairs > 0 && run <= tail_pairs) ? 1 : 0;
+    g.h_tail[L] = (tail_pairs > 0 && (int64_t)run * plan_share <= tail_pairs) ? 1 : 0;
   }
   ST_TRY(g.pairs.grow(sizeof(PairMeta) * npairs, false, ps));
   ST_TRY(g.tail_lv.grow(nlevels, false, ps));
@@ -395,7 +398,9 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   HIP_TRY(hipMemcpyAsync(g.lines_d.p, g.lines.data(), sizeof(int64_t) * (npairs + 1),
                          hipMemcpyHostToDevice, ps));
   HIP_TRY(hipMemcpyAsync(g.tail_lv.p, g.h_tail.data(), nlevels, hipMemcpyHostToDevice, ps));
-  HIP_TRY(hipEventRecord(ev_[0], ps));
+  for (hipEvent_t* e : {&g.ev_plan0, &g.ev_plan1, &g.ev_end})
+    if (!*e) HIP_TRY(hipEventCreate(e));
+  HIP_TRY(hipEventRecord(g.ev_plan0, ps));
 
   PlanArgs pa;
   pa.pool = view();
@@ -428,7 +433,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   HIP_TRY(hipMemcpyAsync(g.h_cells.data(), g.cells.p, sizeof(unsigned long long) * nb,
                          hipMemcpyDeviceToHost, ps));
   HIP_TRY(hipStreamSynchronize(ps));
-  HIP_TRY(hipEventRecord(ev_[1], ps));
+  HIP_TRY(hipEventRecord(g.ev_plan1, ps));
   // Column-blocked tables: ordinal of each bucket's first task, then (with a
   // second short sync, only in plans that have such tables) the arena
   // layout computed on the device.
@@ -483,8 +488,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   return OK;
 }
 
-int Context::launch_levels(Group* const* gs, int ng, std::vector<std::pair<int, int>>* prof,
-                           int lane) {
+int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* prof, int lane) {
   // Launches of one level are independent, and so are levels of different
   // groups: they fork from the main stream onto up to level_streams streams
   // (largest first) and join before the next tick, so latency-bound launches
@@ -576,7 +580,7 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<std::pair<int, 
     }
     if (prof) {
       HIP_TRY(hipEventRecord(prof_ev_[2 * prof->size() + 1], s));
-      prof->emplace_back(w.cls, w.bucket);
+      prof->push_back(ProfLaunch{w.cls, w.bucket, &g});
     }
     ++g.st.launches;
   }
@@ -637,28 +641,6 @@ int Context::sync() {
   return OK;
 }
 
-int Context::finish(Group& g, int32_t* dist, int32_t* const* td_out) {
-  launch_gather_dist(g.pairs.as<PairMeta>(), g.td.as<int32_t>(), g.npairs, g.dist.as<int32_t>(),
-                     stream_);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(dist, g.dist.p, sizeof(int32_t) * g.npairs, hipMemcpyDeviceToHost,
-                         stream_));
-  int32_t aborted[8] = {0};
-  if (g.nxl > 0)
-    HIP_TRY(hipMemcpyAsync(aborted, g.abort.p, sizeof(aborted), hipMemcpyDeviceToHost, stream_));
-  if (td_out) {
-    for (int32_t i = 0; i < g.npairs; ++i) {
-      if (!td_out[i]) continue;
-      const int64_t mn = (int64_t)meta_[g.xs[i]].n * meta_[g.ys[i]].n;
-      HIP_TRY(hipMemcpyAsync(td_out[i], g.td.as<int32_t>() + g.pm[i].td_off, sizeof(int32_t) * mn,
-                             hipMemcpyDeviceToHost, stream_));
-    }
-  }
-  HIP_TRY(hipStreamSynchronize(stream_));
-  if (aborted[0]) return xl_timeout(aborted);
-  return OK;
-}
-
 int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int32_t* dist,
                        int32_t* const* td_out, BatchStats* stats) {
   std::lock_guard<std::mutex> lk(mu_);
@@ -671,39 +653,100 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     if (stats) *stats = BatchStats{};
     return OK;
   }
-  Group& g = batch_;
-  int st = plan(g, npairs, xs, ys);
-  if (st != OK) return st;
-  std::vector<std::pair<int, int>> prof;  // (class, bucket or -1) per timed launch
-  Group* one = &g;
-  while (g.next_level < g.nlevels) {
-    st = launch_levels(&one, 1, profile ? &prof : nullptr);
-    if (st != OK) return st;
+  const int ng = npairs >= batch_split_min
+                     ? std::max(1, std::min<int>(batch_groups, std::min(kMaxLevelStreams, npairs)))
+                     : 1;
+  // Pair i goes to group i % ng; idx[k] maps a group's pairs back.
+  std::vector<std::vector<int32_t>> gx(ng), gy(ng), idx(ng);
+  for (int32_t i = 0; i < npairs; ++i) {
+    gx[i % ng].push_back(xs[i]);
+    gy[i % ng].push_back(ys[i]);
+    idx[i % ng].push_back(i);
   }
-  HIP_TRY(hipEventRecord(ev_[2], stream_));
-  st = finish(g, dist, td_out);
-  if (st != OK) return st;
+  std::vector<ProfLaunch> prof;
+  int st = OK;
+  // Each group: plan (the host waits for its counts while the groups before
+  // it run), then every level, on the group's own stream.
+  for (int k = 0; k < ng; ++k) {
+    Group& g = batch_[k];
+    const int lane = ng > 1 ? k : -1;
+    plan_share = ng;
+    st = plan(g, (int32_t)gx[k].size(), gx[k].data(), gy[k].data(), lane);
+    plan_share = 1;
+    if (st != OK) return st;
+    Group* one = &g;
+    while (g.next_level < g.nlevels) {
+      st = launch_levels(&one, 1, profile ? &prof : nullptr, lane);
+      if (st != OK) return st;
+    }
+    HIP_TRY(hipEventRecord(g.ev_end, lane_stream(lane)));
+  }
+  // Distances, abort records and tables of every group, on its stream.
+  std::vector<std::vector<int32_t>> gd(ng);
+  std::vector<std::array<int32_t, 8>> aborted(ng);
+  for (int k = 0; k < ng; ++k) {
+    Group& g = batch_[k];
+    hipStream_t s = lane_stream(ng > 1 ? k : -1);
+    gd[k].assign(g.npairs, 0);
+    aborted[k].fill(0);
+    launch_gather_dist(g.pairs.as<PairMeta>(), g.td.as<int32_t>(), g.npairs, g.dist.as<int32_t>(),
+                       s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(gd[k].data(), g.dist.p, sizeof(int32_t) * g.npairs,
+                           hipMemcpyDeviceToHost, s));
+    if (g.nxl > 0)
+      HIP_TRY(hipMemcpyAsync(aborted[k].data(), g.abort.p, sizeof(int32_t) * 8,
+                             hipMemcpyDeviceToHost, s));
+    if (td_out) {
+      for (int32_t j = 0; j < g.npairs; ++j) {
+        int32_t* out = td_out[idx[k][j]];
+        if (!out) continue;
+        const int64_t mn = (int64_t)meta_[g.xs[j]].n * meta_[g.ys[j]].n;
+        HIP_TRY(hipMemcpyAsync(out, g.td.as<int32_t>() + g.pm[j].td_off, sizeof(int32_t) * mn,
+                               hipMemcpyDeviceToHost, s));
+      }
+    }
+  }
+  for (int k = 0; k < ng; ++k) HIP_TRY(hipStreamSynchronize(lane_stream(ng > 1 ? k : -1)));
+  for (int k = 0; k < ng; ++k)
+    if (aborted[k][0]) return xl_timeout(aborted[k].data());
+  for (int k = 0; k < ng; ++k)
+    for (size_t j = 0; j < idx[k].size(); ++j) dist[idx[k][j]] = gd[k][j];
+
   if (profile) {
     for (auto& cp : class_profile) cp = ClassProfile{};
     for (size_t i = 0; i < prof.size(); ++i) {
       float ms = 0;
       HIP_TRY(hipEventElapsedTime(&ms, prof_ev_[2 * i], prof_ev_[2 * i + 1]));
-      ClassProfile& cp = class_profile[prof[i].first];
+      ClassProfile& cp = class_profile[prof[i].cls];
+      const Group& g = *prof[i].g;
       cp.launches += 1;
       cp.ms += ms;
-      const int b = prof[i].second;
+      const int b = prof[i].bucket;
       cp.tables += b >= 0 ? (int64_t)g.h_counts[b] : g.leaf_tables;
       cp.cells += b >= 0 ? (int64_t)g.h_cells[b] : g.leaf_cells;
     }
   }
-  float plan_ms = 0, dp_ms = 0;
-  HIP_TRY(hipEventElapsedTime(&plan_ms, ev_[0], ev_[1]));
-  HIP_TRY(hipEventElapsedTime(&dp_ms, ev_[1], ev_[2]));
-  g.st.plan_ms = plan_ms;
-  g.st.dp_ms = dp_ms;
-  g.st.total_ms =
+  // Plan time: the groups' plans added up; DP time: from the end of the
+  // first plan to the end of the last group's levels (later plans overlap).
+  BatchStats out;
+  double dp_ms = 0;
+  for (int k = 0; k < ng; ++k) {
+    const Group& g = batch_[k];
+    float pms = 0, dms = 0;
+    HIP_TRY(hipEventElapsedTime(&pms, g.ev_plan0, g.ev_plan1));
+    HIP_TRY(hipEventElapsedTime(&dms, batch_[0].ev_plan1, g.ev_end));
+    out.plan_ms += pms;
+    dp_ms = std::max(dp_ms, (double)dms);
+    out.cells += g.st.cells;
+    out.tables += g.st.tables;
+    out.levels = std::max(out.levels, g.st.levels);
+    out.launches += g.st.launches;
+  }
+  out.dp_ms = dp_ms;
+  out.total_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-  if (stats) *stats = g.st;
+  if (stats) *stats = out;
   return OK;
 }
 
