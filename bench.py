@@ -219,8 +219,14 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the granularity rule first and use the fastest")
     ap.add_argument("--no-xl", action="store_true", help="skip the 20,000-node pair")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the levels one by one instead of replaying a captured graph")
+    ap.add_argument("--profile-steps", type=int, default=1,
+                    help="steps of the per-launch timed region (class times, roofline)")
     ap.add_argument("--batch-groups", type=int, default=0,
                     help="groups a batch is cut into (0: the library default)")
+    ap.add_argument("--rule", default="",
+                    help='JSON object overriding keys of the granularity rule, e.g. {"max_cpl": 16}')
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="also run the serving leg (Poisson arrivals) for this long")
     ap.add_argument("--serve-rate", type=float, default=50.0)
@@ -260,9 +266,12 @@ def main():
         rule, _ = autotune.tune(ctx, xs, ys, log=log)
     else:
         rule = autotune.load()
+        if args.rule:
+            rule.update({k: int(v) for k, v in json.loads(args.rule).items()})
         autotune.apply(ctx, rule)
     if args.batch_groups:
         ctx.set_batch_groups(args.batch_groups)
+    ctx.set_graphs(not args.no_graphs)
     log(f"granularity rule: {rule}")
 
     dist = None
@@ -270,19 +279,29 @@ def main():
         dist, _, st = ctx.batch(xs, ys)
         log(f"warmup done: {st.tables} tables, {st.cells} cells, {st.levels} levels")
 
-    ctx.set_profile(True)
-    times = []
-    for k in range(args.steps):
-        t = time.perf_counter()
-        d, _, st = ctx.batch(xs, ys)
-        times.append(time.perf_counter() - t)
-        if dist is None:
-            dist = d
-        if not np.array_equal(d, dist):
-            raise SystemExit("batched distances changed between steps")
-        log(f"step {k}: {times[-1] * 1e3:.1f} ms (plan {st.plan_ms:.1f} ms, dp {st.dp_ms:.1f} ms)")
-    prof = ctx.class_profile()
-    ctx.set_profile(False)
+    def timed(steps, profile, tag):
+        ctx.set_profile(profile)
+        times, st = [], None
+        for k in range(steps):
+            t = time.perf_counter()
+            d, _, st = ctx.batch(xs, ys)
+            times.append(time.perf_counter() - t)
+            if not np.array_equal(d, dist):
+                raise SystemExit("batched distances changed between steps")
+            log(f"{tag} {k}: {times[-1] * 1e3:.1f} ms (plan {st.plan_ms:.1f} ms, "
+                f"dp {st.dp_ms:.1f} ms)")
+        prof = ctx.class_profile() if profile else None
+        ctx.set_profile(False)
+        return times, prof, st
+
+    if dist is None:
+        dist, _, _ = ctx.batch(xs, ys)
+    # The timed steps: levels replayed from a captured graph unless
+    # --no-graphs. Then a second timed region with every forest launch
+    # bracketed by HIP events on its own stream (launched one by one): the
+    # per-class device times and the dominant kernel's launch durations.
+    times, _, st = timed(args.steps, False, "step")
+    ptimes, prof, _ = timed(max(1, args.profile_steps), True, "profiled step")
     ms_per_step = 1e3 * sum(times) / len(times)
     pairs_per_s = args.pairs / (ms_per_step / 1e3)
     gcups = st.cells / (ms_per_step / 1e3) / 1e9
@@ -348,6 +367,8 @@ def main():
         "gcups": round(gcups, 2),
         "plan_ms": round(st.plan_ms, 3),
         "dp_ms": round(st.dp_ms, 3),
+        "graphs": not args.no_graphs,
+        "profiled_ms_per_step": round(1e3 * sum(ptimes) / len(ptimes), 3),
         "plain_pairs_per_s": round(plain_pairs_per_s, 2) if n_plain else None,
         "plain_sample_pairs": n_plain,
         "batched_vs_plain": round(pairs_per_s / plain_pairs_per_s, 2) if n_plain else None,

@@ -134,6 +134,13 @@ int xted_set_level_streams(xted_context* ctx, int32_t n);
  * plan overlaps the levels of the groups before it. No counterpart in the
  * reference, which runs one pair at a time. Results do not depend on it. */
 int xted_set_batch_groups(xted_context* ctx, int32_t groups, int32_t split_min);
+/* on = 1 (the default): the level launches of a batch without
+ * column-blocked tables are captured once as a HIP graph and re-launched
+ * while a later batch plans to the same launches (same counts per level and
+ * class, same buffers); 0 launches them one by one. Profiling launches them
+ * one by one regardless. The reference launches every kernel directly
+ * (X-TED_GPU/parallel_X_TED.cu:429-443). Results do not depend on it. */
+int xted_set_graphs(xted_context* ctx, int32_t on);
 
 /* ---- per-class device timing ---------------------------------------------------
  * With profiling on, xted_ted / xted_batch bracket every forest-DP launch

@@ -199,6 +199,32 @@ def test_batch_groups_do_not_change_results(gpu_ctx):
         assert out[4][0][k] == oracle.ted(trees[2 * k], trees[2 * k + 1], threads=4)[0]
 
 
+def test_graph_replay_matches_direct_launches(gpu_ctx):
+    """Levels captured as a graph, the graph re-launched for the same plan
+    (twice) and re-captured after the pool grows: distances and tables equal
+    to direct launches and to the oracle."""
+    trees, xs, ys, shapes = _mixed_batch(gpu_ctx, 24, 3)
+    try:
+        gpu_ctx.set_graphs(False)
+        ref = gpu_ctx.batch(xs, ys, td_shapes=shapes)
+        gpu_ctx.set_graphs(True)
+        runs = [gpu_ctx.batch(xs, ys, td_shapes=shapes) for _ in range(3)]
+        gpu_ctx.pool_add([gen(1, 900, 30, 99, k) for k in range(40)])  # pool arrays move
+        runs.append(gpu_ctx.batch(xs, ys, td_shapes=shapes))
+        # a different batch over the same pool, then the first one again
+        other = gpu_ctx.batch(ys[::-1].copy(), xs[::-1].copy())
+        runs.append(gpu_ctx.batch(xs, ys, td_shapes=shapes))
+    finally:
+        gpu_ctx.set_graphs(True)
+    for out in runs:
+        assert np.array_equal(out[0], ref[0])
+        for a, b in zip(out[1], ref[1]):
+            assert np.array_equal(a, b)
+    for k in range(0, 24, 4):
+        assert ref[0][k] == oracle.ted(trees[2 * k], trees[2 * k + 1], threads=4)[0]
+        assert other[0][23 - k] == oracle.ted(trees[2 * k + 1], trees[2 * k], threads=4)[0]
+
+
 def test_level_streams_do_not_change_results(gpu_ctx):
     """Launches of one level spread over 1, 2 or 4 streams: tables identical,
     distances equal to the oracle's."""

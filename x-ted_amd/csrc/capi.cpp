@@ -251,6 +251,12 @@ int xted_set_batch_groups(xted_context* ctx, int32_t groups, int32_t split_min) 
   return OK;
 }
 
+int xted_set_graphs(xted_context* ctx, int32_t on) {
+  if (!ctx) return ERR_INVALID_ARG;
+  ctx->ctx.use_graphs = on ? 1 : 0;
+  return OK;
+}
+
 int xted_set_profile(xted_context* ctx, int32_t on) {
   if (!ctx) return ERR_INVALID_ARG;
   ctx->ctx.profile = on ? 1 : 0;

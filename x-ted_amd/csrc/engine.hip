@@ -296,6 +296,11 @@ void Group::release() {
     if (*e) (void)hipEventDestroy(*e);
     *e = nullptr;
   }
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  graph_exec = nullptr;
+  graph = nullptr;
+  graph_sig.clear();
   for (DevBuf* b : {&pairs, &blk, &counts, &cells, &cursor, &maxlds, &tickets, &abort, &tasks, &td,
                     &dist, &lines_d, &tail_lv, &xl_off, &xl_total, &xl_words})
     b->release();
@@ -641,6 +646,67 @@ int Context::sync() {
   return OK;
 }
 
+// Everything launch_levels bakes into the launches of a group: the plan's
+// counts and LDS sizes, the buffers, the pool arrays and the streams.
+std::vector<uint64_t> Context::graph_signature(const Group& g, int lane) const {
+  std::vector<uint64_t> s;
+  hipStream_t sl[kMaxLevelStreams];
+  const int ns = lane_streams(lane, sl);
+  s.push_back((uint64_t)ns);
+  for (int k = 0; k < ns; ++k) s.push_back((uint64_t)(uintptr_t)sl[k]);
+  s.push_back((uint64_t)g.npairs);
+  s.push_back((uint64_t)g.nlevels);
+  s.push_back((uint64_t)g.lines[g.npairs]);
+  for (const void* p : {g.pairs.p, g.td.p, g.tasks.p, g.lines_d.p})
+    s.push_back((uint64_t)(uintptr_t)p);
+  const PoolView v = view();
+  const uint64_t* pv = reinterpret_cast<const uint64_t*>(&v);
+  s.insert(s.end(), pv, pv + sizeof(PoolView) / sizeof(uint64_t));
+  s.insert(s.end(), g.h_counts.begin(), g.h_counts.end());
+  for (int32_t m : g.h_maxlds) s.push_back((uint64_t)(uint32_t)m);
+  return s;
+}
+
+// All levels of a planned group on its lane: as launches, or as a graph.
+int Context::run_levels(Group& g, int lane, std::vector<ProfLaunch>* prof) {
+  Group* one = &g;
+  if (!use_graphs || prof || g.nxl > 0) {
+    while (g.next_level < g.nlevels) {
+      const int st = launch_levels(&one, 1, prof, lane);
+      if (st != OK) return st;
+    }
+    return OK;
+  }
+  hipStream_t ls = lane_stream(lane);
+  std::vector<uint64_t> sig = graph_signature(g, lane);
+  if (!g.graph_exec || sig != g.graph_sig) {
+    if (g.graph_exec) HIP_TRY(hipGraphExecDestroy(g.graph_exec));
+    if (g.graph) HIP_TRY(hipGraphDestroy(g.graph));
+    g.graph_exec = nullptr;
+    g.graph = nullptr;
+    g.graph_sig.clear();
+    HIP_TRY(hipStreamBeginCapture(ls, hipStreamCaptureModeThreadLocal));
+    int st = OK;
+    while (st == OK && g.next_level < g.nlevels) st = launch_levels(&one, 1, nullptr, lane);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ls, &graph);
+    if (st != OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return st;
+    }
+    HIP_TRY(e);
+    g.graph = graph;
+    HIP_TRY(hipGraphInstantiate(&g.graph_exec, g.graph, nullptr, nullptr, 0));
+    g.graph_sig = std::move(sig);
+    g.graph_launches = g.st.launches;
+  } else {
+    g.next_level = g.nlevels;
+    g.st.launches = g.graph_launches;
+  }
+  HIP_TRY(hipGraphLaunch(g.graph_exec, ls));
+  return OK;
+}
+
 int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int32_t* dist,
                        int32_t* const* td_out, BatchStats* stats) {
   std::lock_guard<std::mutex> lk(mu_);
@@ -674,11 +740,8 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     st = plan(g, (int32_t)gx[k].size(), gx[k].data(), gy[k].data(), lane);
     plan_share = 1;
     if (st != OK) return st;
-    Group* one = &g;
-    while (g.next_level < g.nlevels) {
-      st = launch_levels(&one, 1, profile ? &prof : nullptr, lane);
-      if (st != OK) return st;
-    }
+    st = run_levels(g, lane, profile ? &prof : nullptr);
+    if (st != OK) return st;
     HIP_TRY(hipEventRecord(g.ev_end, lane_stream(lane)));
   }
   // Distances, abort records and tables of every group, on its stream.

@@ -73,6 +73,12 @@ struct Group {
   int32_t pin_cap = 0;
   hipEvent_t done_ev = nullptr;
   hipEvent_t ev_plan0 = nullptr, ev_plan1 = nullptr, ev_end = nullptr;  // batch timing
+  // The group's level launches captured as a graph, and the launch inputs
+  // (plan counts, buffers, streams) it was captured for.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uint64_t> graph_sig;
+  int32_t graph_launches = 0;
   BatchStats st;
   Group() = default;
   Group(const Group&) = delete;
@@ -160,6 +166,10 @@ class Context {
   // as a tail level by the pairs of the whole batch that reach it
   // (estimated as the group's count times this).
   int32_t plan_share = 1;
+  // Level launches of a batch without column-blocked tables run as a
+  // captured graph (re-launched while the plan's launches stay the same);
+  // not while profiling, which times each launch with events.
+  int32_t use_graphs = 1;
   ClassProfile class_profile[kNumClasses];
 
   hipStream_t stream() const { return stream_; }
@@ -185,6 +195,8 @@ class Context {
   uint32_t xl_epoch_ = 0;  // tag of the latest column-blocked launch
   std::vector<hipEvent_t> prof_ev_;
   int grow_pool(int64_t nodes);
+  std::vector<uint64_t> graph_signature(const Group& g, int lane) const;
+  int run_levels(Group& g, int lane, std::vector<ProfLaunch>* prof);
   PoolView view() const;
 };
 

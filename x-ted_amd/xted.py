@@ -68,6 +68,7 @@ _SIGNATURES = {
     "xted_set_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_level_streams": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_batch_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "xted_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_tail_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_xl_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_class_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
@@ -278,6 +279,9 @@ class Context:
 
     def set_batch_groups(self, groups, split_min=256):
         _check(lib().xted_set_batch_groups(self._h, groups, split_min))
+
+    def set_graphs(self, on=True):
+        _check(lib().xted_set_graphs(self._h, 1 if on else 0))
 
     def set_profile(self, on=True):
         _check(lib().xted_set_profile(self._h, 1 if on else 0))
