@@ -141,7 +141,8 @@ def xl_pair_bench(reps=3):
             runs.append((time.perf_counter() - t, st, int(d[0])))
         prof = ctx.class_profile()
     wall, st, dist = sorted(runs, key=lambda r: r[0])[len(runs) // 2]
-    xl = {xted.class_name(c): round(v[3], 2) for c, v in prof.items() if c >= 27}
+    xl = {xted.class_name(c): round(v[3], 2) for c, v in prof.items()
+          if 27 <= c < 27 + len(xted.XL_BW)}
     return {"pair": "rrt20000_a500", "dist": dist, "ms": round(wall * 1e3, 2),
             "gcups": round(st.cells / wall / 1e9, 2),
             "hbm_gbs_algorithmic": round(BYTES_PER_CELL * st.cells / wall / 1e9, 1),

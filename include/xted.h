@@ -42,7 +42,9 @@ extern "C" {
  * of 4096, 2048, 1024, 512, 256 or 128 columns, one workgroup per block, for
  * tables too wide or too LDS-hungry for one workgroup (the role of the
  * reference's multi-block and last-table kernels,
- * X-TED_GPU/parallel_X_TED.cu:883-1131). */
+ * X-TED_GPU/parallel_X_TED.cu:883-1131); 33 = tables of at most 16 columns,
+ * four to a wave (the role of the reference's single-thread tasks,
+ * X-TED_GPU/parallel_X_TED.cu:382). */
 #define XTED_CLASS_AUTO (-1)
 #define XTED_CLASS_LANE 0
 
@@ -110,8 +112,9 @@ int xted_batch(xted_context* ctx, int32_t npairs, const int32_t* x_ids, const in
  * least wide_cols columns get at least min_waves waves. */
 int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t max_cpl,
                     int32_t min_waves, int32_t wide_cols);
-/* Forces every table that fits into class cls, a row-scan class (1..25) or
- * a column-blocked one (27..32); XTED_CLASS_AUTO restores the rule.
+/* Forces every table that fits into class cls, a row-scan class (1..25), a
+ * column-blocked one (27..32) or the tiny form (33); XTED_CLASS_AUTO
+ * restores the rule.
  * lane_enable = 0 sends single-row tables to the row-scan kernels. */
 int xted_force_class(xted_context* ctx, int32_t cls, int32_t lane_enable);
 /* Levels of a batch that at most tail_pairs pairs reach (the last levels of

@@ -59,7 +59,7 @@ def test_config2_mixed_shapes(gpu_ctx):
     assert_same(gpu_ctx, y, x, threads=16)
 
 
-@pytest.mark.parametrize("cls", list(range(1, 26)))
+@pytest.mark.parametrize("cls", list(range(1, 26)) + [xted.CLASS_TINY])
 def test_every_row_class_forced(gpu_ctx, cls):
     """Every table that fits goes through one (waves, columns-per-lane) class;
     single-row tables too (lane kernel off). Results must not change a bit."""

@@ -8,8 +8,8 @@
 //                  (keyroots in descending preorder; K <= n)
 //   sorted keyroots skr/skd/ssize/sleaf       [node_off + k], k in [0, K):
 //                  the K1 leaf keyroots first, then the others by ascending
-//                  nesting height d, those of height d at
-//                  [K1 + kdoff[d], K1 + kdoff[d+1])
+//                  nesting height d (and size band within a height), those
+//                  of height d at [K1 + kdoff[d], K1 + kdoff[d+1])
 //   spre           [node_off + k], k in [0, K]: prefix sums of ssize
 //   kdoff          [node_off + d], d in [0, kdmax+1]
 // A pair owns a tree-distance table TD (m x n, row-major, preorder) and its
@@ -83,12 +83,17 @@ XTED_HD inline uint32_t task_j0(uint64_t t) { return (uint32_t)t & 0xFFFFFu; }
 // Classes 27..32: column-blocked form for tables too wide or too LDS-hungry
 //   for one workgroup: blocks of xl_bw(k) columns, one workgroup each,
 //   chained per row through device memory, class = 27 + k.
+// Class 33: tables of at most kTinyCols columns, four to a wave (one per
+//   16-lane row of it).
 constexpr int kClassLane = 0;  // the leaf class
 constexpr int kNumRowClasses = 25;
 constexpr int kClassXL = 26;
 constexpr int kFirstXLClass = 27;
 constexpr int kNumXLForms = 6;
-constexpr int kNumClasses = kFirstXLClass + kNumXLForms;  // 33
+constexpr int kClassTiny = kFirstXLClass + kNumXLForms;  // 33
+constexpr int kNumClasses = kClassTiny + 1;              // 34
+constexpr int kTinyCols = 16;
+constexpr int kTinyMaxLds = 2048;  // slot bytes of one tiny table (16 of them per workgroup)
 // Column-blocked forms: waves and columns per lane of one block.
 XTED_HD inline int xl_waves(int k) { return k <= 1 ? 8 : k == 2 ? 4 : 2; }
 XTED_HD inline int xl_cpl(int k) { return k == 0 ? 8 : k <= 3 ? 4 : k == 4 ? 2 : 1; }
@@ -121,6 +126,7 @@ struct ClassRule {
   int32_t lds_max;       // slot bytes one workgroup may hold (device limit)
   int32_t max_cpl;       // most columns per lane before a table gets more waves
   int32_t xl_seq_rows = kXLSeqRows;  // column-blocked tables this short: one workgroup
+  int32_t tiny = 1;                  // tables of <= kTinyCols columns take the tiny form
 };
 
 XTED_HD inline int ilog2_ceil(int v) {
@@ -161,6 +167,7 @@ XTED_HD inline int table_class(int a, int b, int64_t lds_need, int kd_r, int kd_
     }
     return kClassXL;
   }
+  if (rule.tiny && cols <= kTinyCols && lds_need <= kTinyMaxLds) return kClassTiny;
   int wi = 0;
   while (wi < 4 && lds_need > (int64_t)rule.lds_per_wave * (1 << wi)) ++wi;
   while (wi < 4 && cols > 64 * rule.max_cpl * (1 << wi)) ++wi;

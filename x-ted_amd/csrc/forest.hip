@@ -27,6 +27,7 @@
 
 #include <climits>
 #include <type_traits>
+#include <utility>
 
 #include "kernels.hpp"
 
@@ -816,6 +817,150 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
 #undef TCOL
 }
 
+// ---------------------------------------------------------------------------
+// Tiny tables (at most 16 columns, so at most 16 rows): four per wave, one
+// per 16-lane DPP row, one column per lane (t = lane & 15). A 16-lane row is
+// what row_shr scans and row_newbcast broadcasts over, so a table needs no
+// barrier and no LDS exchange; lane k of a table holds the metadata of its
+// row k and row k takes it with row_newbcast:k; the TD values of all rows
+// are read up front. Tables of one wave run to the longest one's row
+// count, each lane storing only while its table has rows left.
+// ---------------------------------------------------------------------------
+template <class F, int... K>
+__device__ __forceinline__ void static_for(F& f, std::integer_sequence<int, K...>) {
+  (f(std::integral_constant<int, K>{}), ...);
+}
+
+constexpr int kTinySeg = 16;
+constexpr int kTinyPerGroup = 256 / kTinySeg;  // tables per workgroup of 4 waves
+#define DPP_ROW_NEWBCAST(n) (0x150 + (n))
+
+__device__ __forceinline__ int seg16_incl_min(int v) {
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(1), 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(2), 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(4), 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(8), 0xf, 0xf, false));
+  return v;
+}
+
+__global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) {
+  extern __shared__ uint16_t tslots[];
+  __shared__ uint16_t sink[256];
+  const int t = threadIdx.x & (kTinySeg - 1);  // this lane's column, t = cmax-1-c
+  const int seg = threadIdx.x / kTinySeg;      // table of this lane in the workgroup
+  const int64_t tix = (int64_t)blockIdx.x * kTinyPerGroup + seg;
+  const bool live = tix < A.ntasks;
+  const uint64_t task = A.tasks[live ? tix : 0];  // idle segments redo task 0, storing nothing
+  const PairMeta pm = A.pairs[task_pair(task)];
+  const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
+  const int32_t i0 = (int32_t)task_i0(task), j0 = (int32_t)task_j0(task);
+  const int32_t a = A.pool.orl[tx.node_off + i0] - i0 + 1;
+  const int32_t b = A.pool.orl[ty.node_off + j0] - j0 + 1;
+  const bool rows_x = a <= b;
+  const int64_t offR = rows_x ? tx.node_off : ty.node_off;
+  const int64_t offC = rows_x ? ty.node_off : tx.node_off;
+  const int32_t* __restrict__ orlR = A.pool.orl + offR;
+  const int32_t* __restrict__ labR = A.pool.label + offR;
+  const int32_t* __restrict__ kdepR = A.pool.kdep + offR;
+  const int32_t* __restrict__ orlC = A.pool.orl + offC;
+  const int32_t* __restrict__ labC = A.pool.label + offC;
+  const int32_t* __restrict__ lrankC = A.pool.lrank + offC;
+  const int32_t r0 = rows_x ? i0 : j0, c0 = rows_x ? j0 : i0;
+  int32_t* TDo = A.td + (rows_x ? pm.td_off : pm.tdt_off);  // [r][c]
+  int32_t* TDt = A.td + (rows_x ? pm.tdt_off : pm.td_off);  // [c][r]
+  const int64_t ldo = rows_x ? ty.n : tx.n;
+  const int64_t ldt = rows_x ? tx.n : ty.n;
+  const int32_t rmax = orlR[r0] + 1, cmax = orlC[c0] + 1;
+  const int32_t ncols = cmax - c0, nrows = rmax - r0;
+  const int32_t base = kdepR[r0];
+  const int32_t lbase = lrankC[c0];
+  const int32_t wb = lrankC[cmax] - lbase - 1;
+  const int32_t rowlen = wb + 1;
+  uint16_t* slots = tslots + seg * (2 * slot_ints);
+
+  // This lane's column (as in the row scan, one element).
+  const int32_t c = max(cmax - 1 - t, c0);
+  const int32_t oc = orlC[c];
+  const int32_t labc = labC[c];
+  const int32_t ocm = orlC[max(c - 1, c0)];
+  const bool tree_col = oc == cmax - 1;
+  const int32_t bfar = tree_col ? -1 : cmax - oc - 1;
+  const int32_t foff = tree_col ? wb : lrankC[oc] - lbase;
+  const bool col_ok = live && t < ncols;
+  const bool writes = col_ok && c > c0 && ocm == c - 1;
+  uint16_t* const wdst_rel = writes ? slots + (lrankC[max(c - 1, c0)] - lbase) : sink + threadIdx.x;
+  int32_t negt = -t;
+  asm("" : "+v"(negt));
+
+  // Lane t's metadata: row rmax-1-t (rows past the table read row r0).
+  int32_t mlab, mrf, mslot;
+  {
+    const int32_t r = max(rmax - 1 - t, r0);
+    const int32_t rp = max(r - 1, r0);
+    const int32_t orr = orlR[r], kd = kdepR[r];
+    const int32_t orp = orlR[rp], kdp = kdepR[rp];
+    const int32_t fs = (orr + 1 < rmax) ? kd - base - 1 : 0;
+    const bool saved = r > r0 && orp == r - 1;
+    mlab = labR[r];
+    mrf = orr + 1;
+    mslot = fs | (saved ? (((kdp - base - 1) << 16) | (int32_t)0x80000000) : 0);
+  }
+  // TD of this lane's column for every row (row k at r = rmax-1-k).
+  int32_t tdv[kTinySeg];
+#pragma unroll
+  for (int k = 0; k < kTinySeg; ++k)
+    tdv[k] = k < nrows ? TDo[(int64_t)(rmax - 1 - k) * ldo + c] : 0;
+
+  // Rows of the longest table of this wave.
+  const int32_t nmax =
+      max(max(__builtin_amdgcn_readlane(nrows, 0), __builtin_amdgcn_readlane(nrows, 16)),
+          max(__builtin_amdgcn_readlane(nrows, 32), __builtin_amdgcn_readlane(nrows, 48)));
+
+  int32_t Fpm = 2;          // F[r+1][c] + 1 - t
+  int32_t left_prev = 0;    // F[r+1][cmax]
+  // Row k (a compile-time index: it selects the DPP broadcast lane).
+  auto row = [&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k >= nmax) return;  // uniform
+    const bool act = k < nrows;
+    const int32_t r = rmax - 1 - k;
+    const int32_t labr = __builtin_amdgcn_update_dpp(0, mlab, DPP_ROW_NEWBCAST(k), 0xf, 0xf, false);
+    const int32_t Rf = __builtin_amdgcn_update_dpp(0, mrf, DPP_ROW_NEWBCAST(k), 0xf, 0xf, false);
+    const int32_t sl = __builtin_amdgcn_update_dpp(0, mslot, DPP_ROW_NEWBCAST(k), 0xf, 0xf, false);
+    const bool tree_row = Rf == rmax;
+    // F[r+1][c+1] - t: the left neighbour's previous F (lane t-1), or the
+    // boundary F[r+1][cmax] for t = 0.
+    const int32_t fpr = __builtin_amdgcn_update_dpp(left_prev, Fpm - 1 - negt, DPP_ROW_SHR(1),
+                                                    0xf, 0xf, false);
+    const int32_t diag = fpr + negt + (int32_t)(labr != labc);
+    const int32_t sv = slots[(sl & 0xFFFF) * rowlen + foff];
+    const int32_t far = (tree_row ? bfar : sv) + tdv[k] + negt;
+    const int32_t dm = tree_row && tree_col ? diag : far;
+    const int32_t carry = rmax - r + 1;
+    const int32_t h = min(carry, seg16_incl_min(min(dm, Fpm)));  // F - t
+    if (act) {
+      Fpm = h + 1;
+      left_prev = rmax - r;
+      if (tree_row && tree_col && col_ok) {
+        TDo[(int64_t)r * ldo + c] = h + t;
+        TDt[(int64_t)c * ldt + r] = h + t;
+      }
+      if (sl < 0) {
+        uint16_t* wrow = slots + ((sl >> 16) & 0x7FFF) * rowlen;
+        *(writes ? wdst_rel + (wrow - slots) : wdst_rel) = (uint16_t)(h + t);
+        if (t == 0) wrow[wb] = (uint16_t)(rmax - r);
+      }
+    }
+  };
+  static_for(row, std::make_integer_sequence<int, kTinySeg>{});
+}
+
+void launch_tiny(const DPArgs& a, int slot_bytes, hipStream_t s) {
+  const int slot_ints = (slot_bytes + 3) / 4;  // per table, in 4-byte units
+  hipLaunchKernelGGL(tiny_kernel, dim3((unsigned)((a.ntasks + kTinyPerGroup - 1) / kTinyPerGroup)),
+                     dim3(256), (size_t)kTinyPerGroup * slot_ints * 4, s, a, slot_ints);
+}
+
 // Arena offsets of every column-blocked task of a batch, in bucket order
 // (level, then form), and their total (one workgroup).
 __global__ __launch_bounds__(1024) void xl_layout_kernel(XLLayoutArgs A) {
@@ -933,6 +1078,10 @@ hipError_t forest_init_attributes(int limit) {
 // Launches one (class, task list). Returns false for a class without a kernel.
 bool launch_forest(int cls, const DPArgs& a, int lds_bytes, hipStream_t s) {
   if (a.ntasks <= 0) return true;
+  if (cls == kClassTiny) {
+    launch_tiny(a, lds_bytes, s);
+    return true;
+  }
   if (cls >= 1 && cls <= kNumRowClasses) {
     const int wi = (cls - 1) / 5, ci = (cls - 1) % 5;
     switch (wi) {

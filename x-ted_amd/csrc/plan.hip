@@ -7,8 +7,9 @@
 // thrust::sort_by_key / filter_new with several host syncs per level, and
 // X-TED_GPU/function.cu:18-57). Here the whole plan is built once per batch
 // by three kernels (count, scan, scatter) and read back with one copy of the
-// bucket counts. Appends use wave-aggregated atomics: each distinct bucket in
-// a wave costs one atomic, and a wave's tables land contiguously.
+// bucket counts. Counts and appends are aggregated per wave and then per
+// workgroup: each distinct bucket of a workgroup costs one global atomic, and
+// a wave's tables of one bucket land contiguously.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -48,7 +49,8 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
   int cls = table_class(a, b, need, a <= b ? kdx : kdy, a <= b ? kdy : kdx, leaves_c,
                         A.lane_enable != 0, rule, &need);
   // A forced column-blocked form takes every multi-row table it can hold.
-  if (A.force_class >= kFirstXLClass && cls >= 1 && cls != kClassXL) {
+  if (A.force_class >= kFirstXLClass && A.force_class < kFirstXLClass + kNumXLForms && cls >= 1 &&
+      cls != kClassXL) {
     const int k = A.force_class - kFirstXLClass;
     const int32_t kd_c = a <= b ? kdy : kdx;
     const int64_t n = xl_lds_need(a <= b ? kdx : kdy, kd_c, leaves_c, xl_bw(k));
@@ -57,12 +59,23 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
       need = n;
     }
   }
+  // A forced row-scan form (or the tiny form) takes every on-chip table it
+  // can hold.
+  if (A.force_class == kClassTiny && cls >= 1 && (cls <= kNumRowClasses || cls == kClassTiny)) {
+    const int cols = a <= b ? b : a;
+    const int64_t row_need = (int64_t)slots * leaves_c * 2;
+    if (cols <= kTinyCols && row_need <= kTinyMaxLds) {
+      cls = kClassTiny;
+      need = row_need;
+    }
+  }
   if (A.force_class >= 1 && A.force_class <= kNumRowClasses && cls >= 1 &&
-      cls <= kNumRowClasses) {
+      (cls <= kNumRowClasses || cls == kClassTiny)) {
     const int wi = (A.force_class - 1) / 5, ci = (A.force_class - 1) % 5;
     const int cols = a <= b ? b : a;
     const int64_t row_need = (int64_t)slots * leaves_c * 2;
-    if (cls < kClassXL && cols <= 64 * (1 << wi) * (1 << ci) && row_need <= A.rule.lds_max) {
+    if ((cls < kClassXL || cls == kClassTiny) && cols <= 64 * (1 << wi) * (1 << ci) &&
+        row_need <= A.rule.lds_max) {
       cls = A.force_class;
       need = row_need;
     }
@@ -98,35 +111,70 @@ __device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
   return __builtin_amdgcn_readlane(v, 63);
 }
 
-// Scatter pass: this lane's slot in cursors[bucket], after one
-// wave-aggregated atomicAdd per distinct bucket among the active lanes.
-__device__ __forceinline__ unsigned long long wave_reserve(unsigned long long* cursors, int bucket,
-                                                           bool active) {
-  unsigned long long pos = 0;
-  const unsigned long long lt = __lanemask_lt();
-  unsigned long long todo = __ballot(active);
-  const int lane = (int)(threadIdx.x & 63);
-  while (todo) {
-    const int leader = __ffsll((long long)todo) - 1;
-    const int lb = __shfl(bucket, leader);
-    const bool mine = active && bucket == lb;
-    const unsigned long long same = __ballot(mine);
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(&cursors[lb], (unsigned long long)__popcll(same));
-    base = __shfl(base, leader);
-    if (mine) pos = base + __popcll(same & lt);
-    todo &= ~same;
+// Per-workgroup bucket table in LDS: the plan's global atomics go to a few
+// thousand (level, class) buckets and device-scope atomics are performed
+// memory-side, so a wave's per-bucket partials are first combined per
+// workgroup (open addressing on the bucket id), and each workgroup then
+// makes one global atomic per bucket it touched. A bucket that finds the
+// table full takes the global atomics directly.
+constexpr int kHTBits = 8;
+constexpr int kHT = 1 << kHTBits;
+struct PlanLds {
+  int32_t key[kHT];  // bucket id, or -1
+  uint32_t cnt[kHT];
+  int32_t lds[kHT];
+  unsigned long long cells[kHT];  // count pass: cells; scatter pass: base
+};
+
+__device__ __forceinline__ int ht_slot(PlanLds& T, int bucket, bool insert) {
+  int h = (int)(((uint32_t)bucket * 0x9E3779B1u) >> (32 - kHTBits));
+  for (int probe = 0; probe < kHT; ++probe) {
+    const int k = insert ? atomicCAS(&T.key[h], -1, bucket) : T.key[h];
+    if (k == bucket || (insert && k == -1)) return h;
+    if (k == -1) return -1;  // lookup: absent
+    h = (h + 1) & (kHT - 1);
   }
-  return pos;
+  return -1;  // full
 }
 
-// Count pass: per distinct bucket of the wave, one fire-and-forget atomic
-// each for the table count, the interior cells (a table has < 2^30 cells;
-// the wave sum is taken over 16-bit halves so 32-bit sums cannot overflow)
-// and the largest slot-storage need of the bucket. All three arrays are
-// indexed by bucket, so the atomics of different buckets fall on different
-// cache lines (a per-class maximum put every wave of the chip on one line).
-__device__ __forceinline__ void wave_count(const PlanArgs& A, int bucket, bool active,
+__device__ __forceinline__ void ht_clear(PlanLds& T) {
+  for (int i = threadIdx.x; i < kHT; i += kPlanThreads) {
+    T.key[i] = -1;
+    T.cnt[i] = 0;
+    T.lds[i] = 0;
+    T.cells[i] = 0;
+  }
+}
+
+// The tables of this workgroup: y keyroots [l_lo, l_lo + rows) of the
+// pair's sorted list against all Kx x keyroots, total of them.
+struct PlanChunk {
+  int p, kbase, lbase, Kx, l_lo;
+  int64_t total;
+};
+__device__ __forceinline__ PlanChunk plan_chunk(const PlanArgs& A, const TreeMeta& tx,
+                                                const TreeMeta& ty, int p) {
+  PlanChunk c;
+  c.p = p;
+  // With the leaf kernel on, tables with a leaf keyroot are not planned here
+  // (the leaf kernel writes them): only the non-leaf keyroots [K1, K) of the
+  // sorted lists take part.
+  c.kbase = A.lane_enable ? tx.K1 : 0;
+  c.lbase = A.lane_enable ? ty.K1 : 0;
+  c.Kx = tx.K - c.kbase;
+  const int Ky = ty.K - c.lbase;
+  const int lper = max(1, A.lchunk_tables / max(c.Kx, 1));  // same as plan_blocks()
+  c.l_lo = (int)(blockIdx.x - A.blk_start[p]) * lper;
+  const int l_hi = min(Ky, c.l_lo + lper);
+  c.total = (int64_t)max(0, l_hi - c.l_lo) * c.Kx;
+  return c;
+}
+
+// Count pass: per distinct bucket of a wave, the table count, the interior
+// cells (a table has < 2^30 cells; the wave sum is taken over 16-bit halves
+// so 32-bit sums cannot overflow) and the largest slot-storage need, added
+// into the workgroup's table.
+__device__ __forceinline__ void wave_count(const PlanArgs& A, PlanLds& T, int bucket, bool active,
                                            uint32_t cells, int32_t lds) {
   unsigned long long todo = __ballot(active);
   const int lane = (int)(threadIdx.x & 63);
@@ -139,45 +187,118 @@ __device__ __forceinline__ void wave_count(const PlanArgs& A, int bucket, bool a
     const uint32_t hi = wave_sum_u32(mine ? (cells >> 16) : 0u);
     const int32_t mx = wave_max_i32(mine ? lds : 0);
     if (lane == leader) {
-      atomicAdd(&A.counts[lb], (unsigned long long)__popcll(same));
-      atomicAdd(&A.cells[lb], ((unsigned long long)hi << 16) + lo);
-      if (mx > 0) atomicMax(&A.maxlds[lb], mx);
+      const unsigned long long n = (unsigned long long)__popcll(same);
+      const unsigned long long sum = ((unsigned long long)hi << 16) + lo;
+      const int h = ht_slot(T, lb, true);
+      if (h >= 0) {
+        atomicAdd(&T.cnt[h], (uint32_t)n);
+        atomicAdd(&T.cells[h], sum);
+        atomicMax(&T.lds[h], mx);
+      } else {
+        atomicAdd(&A.counts[lb], n);
+        atomicAdd(&A.cells[lb], sum);
+        if (mx > 0) atomicMax(&A.maxlds[lb], mx);
+      }
     }
     todo &= ~same;
   }
 }
 
+// Scatter pass, first sweep: tables per distinct bucket into the table.
+__device__ __forceinline__ void wave_tally(PlanLds& T, int bucket, bool active) {
+  unsigned long long todo = __ballot(active);
+  const int lane = (int)(threadIdx.x & 63);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lb = __shfl(bucket, leader);
+    const unsigned long long same = __ballot(active && bucket == lb);
+    if (lane == leader) {
+      const int h = ht_slot(T, lb, true);
+      if (h >= 0) atomicAdd(&T.cnt[h], (uint32_t)__popcll(same));
+    }
+    todo &= ~same;
+  }
+}
+
+// Scatter pass, second sweep: this lane's slot, from the workgroup's range
+// of the bucket (or, for a bucket not in the table, a global reservation).
+__device__ __forceinline__ unsigned long long wave_reserve(const PlanArgs& A, PlanLds& T,
+                                                           int bucket, bool active) {
+  unsigned long long pos = 0;
+  const unsigned long long lt = __lanemask_lt();
+  unsigned long long todo = __ballot(active);
+  const int lane = (int)(threadIdx.x & 63);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lb = __shfl(bucket, leader);
+    const bool mine = active && bucket == lb;
+    const unsigned long long same = __ballot(mine);
+    unsigned long long base = 0;
+    if (lane == leader) {
+      const uint32_t n = (uint32_t)__popcll(same);
+      const int h = ht_slot(T, lb, false);
+      base = h >= 0 ? T.cells[h] + atomicAdd(&T.cnt[h], n)
+                    : atomicAdd(&A.cursor[lb], (unsigned long long)n);
+    }
+    base = __shfl(base, leader);
+    if (mine) pos = base + __popcll(same & lt);
+    todo &= ~same;
+  }
+  return pos;
+}
+
 template <bool kScatter>
 __global__ __launch_bounds__(kPlanThreads) void plan_kernel(PlanArgs A) {
-  const int64_t blk = blockIdx.x;
-  const int p = find_pair(A.blk_start, A.npairs, blk);
+  __shared__ PlanLds T;
+  const int p = find_pair(A.blk_start, A.npairs, blockIdx.x);
   const PairMeta pm = A.pairs[p];
   const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
-  // With the lane kernel on, tables with a leaf keyroot are not planned here
-  // (the lane kernel enumerates them per level): only the non-leaf keyroots
-  // [K1, K) of the sorted lists take part.
-  const int kbase = A.lane_enable ? tx.K1 : 0, lbase = A.lane_enable ? ty.K1 : 0;
-  const int Kx = tx.K - kbase, Ky = ty.K - lbase;
-  const int lper = max(1, A.lchunk_tables / max(Kx, 1));  // same as plan_blocks()
-  const int l_lo = (int)(blk - A.blk_start[p]) * lper;
-  const int l_hi = min(Ky, l_lo + lper);
-  const int64_t total = (int64_t)max(0, l_hi - l_lo) * Kx;
-  for (int64_t base = 0; base < total; base += kPlanThreads) {
-    const int64_t idx = base + threadIdx.x;
-    const bool active = idx < total;
-    int bucket = 0, lds = 0;
-    uint32_t i0 = 0, j0 = 0;
-    unsigned long long cells = 0;
-    if (active) {
-      const int l = lbase + l_lo + (int)(idx / Kx), k = kbase + (int)(idx % Kx);
-      classify(A, tx, ty, k, l, &bucket, &lds, &i0, &j0, &cells);
+  const PlanChunk ch = plan_chunk(A, tx, ty, p);
+  ht_clear(T);
+  __syncthreads();
+  auto sweep = [&](auto body) {
+    for (int64_t base = 0; base < ch.total; base += kPlanThreads) {
+      const int64_t idx = base + threadIdx.x;
+      const bool active = idx < ch.total;
+      int bucket = 0, lds = 0;
+      uint32_t i0 = 0, j0 = 0;
+      unsigned long long cells = 0;
+      if (active) {
+        const int l = ch.lbase + ch.l_lo + (int)(idx / ch.Kx), k = ch.kbase + (int)(idx % ch.Kx);
+        classify(A, tx, ty, k, l, &bucket, &lds, &i0, &j0, &cells);
+      }
+      body(active, bucket, lds, i0, j0, cells);
     }
-    if (kScatter) {
-      const unsigned long long pos = wave_reserve(A.cursor, bucket, active);
-      if (active) A.tasks[pos] = task_pack((uint32_t)p, i0, j0);
-    } else {
-      wave_count(A, bucket, active, (uint32_t)cells, lds);
+  };
+  if (!kScatter) {
+    sweep([&](bool active, int bucket, int lds, uint32_t, uint32_t, unsigned long long cells) {
+      wave_count(A, T, bucket, active, (uint32_t)cells, lds);
+    });
+    __syncthreads();
+    for (int h = threadIdx.x; h < kHT; h += kPlanThreads) {
+      const int b = T.key[h];
+      if (b < 0) continue;
+      atomicAdd(&A.counts[b], (unsigned long long)T.cnt[h]);
+      atomicAdd(&A.cells[b], T.cells[h]);
+      if (T.lds[h] > 0) atomicMax(&A.maxlds[b], T.lds[h]);
     }
+  } else {
+    sweep([&](bool active, int bucket, int, uint32_t, uint32_t, unsigned long long) {
+      wave_tally(T, bucket, active);
+    });
+    __syncthreads();
+    // One range per bucket for the workgroup; cnt becomes its cursor.
+    for (int h = threadIdx.x; h < kHT; h += kPlanThreads) {
+      const int b = T.key[h];
+      if (b < 0) continue;
+      T.cells[h] = atomicAdd(&A.cursor[b], (unsigned long long)T.cnt[h]);
+      T.cnt[h] = 0;
+    }
+    __syncthreads();
+    sweep([&](bool active, int bucket, int, uint32_t i0, uint32_t j0, unsigned long long) {
+      const unsigned long long pos = wave_reserve(A, T, bucket, active);
+      if (active) A.tasks[pos] = task_pack((uint32_t)ch.p, i0, j0);
+    });
   }
 }
 

@@ -206,30 +206,38 @@ __global__ __launch_bounds__(kPrepThreads) void tree_prep_kernel(PrepArgs A) {
 
   // ---- keyroots sorted for the scheduler -----------------------------------
   // Leaf keyroots first, then the other keyroots by ascending nesting height
-  // (a stable counting sort over the descending list). One thread: this runs
-  // once per tree at upload.
+  // and, within a height, by subtree size band (<= 16, 64, 256, 1024, more
+  // nodes): a stable counting sort over the descending list. The plan gives
+  // a wave consecutive keyroots of this order, so its tables fall into few
+  // (level, class) buckets. One thread: this runs once per tree at upload.
   if (tid == 0) {
+    constexpr int kBands = 5;
+    auto band = [](int32_t sz) { return sz <= 16 ? 0 : sz <= 64 ? 1 : sz <= 256 ? 2 : sz <= 1024 ? 3 : 4; };
     const int32_t kdmax = kdn[0];
-    int32_t* hist = anc0;  // free scratch, kdmax+2 <= n+1 entries
-    for (int d = 0; d <= kdmax + 1; ++d) hist[d] = 0;
+    // (kdmax+1)*kBands <= 5(n+1) entries: anc0 .. flag are free here.
+    int32_t* hist = anc0;
+    for (int d = 0; d < (kdmax + 1) * kBands; ++d) hist[d] = 0;
     int32_t K1 = 0;
     for (int k = 0; k < K; ++k) {
       const int32_t i = A.kr[off + k];
       if (orl[i] == i) ++K1;
-      else ++hist[kdn[i]];
+      else ++hist[kdn[i] * kBands + band(orl[i] - i + 1)];
     }
     int32_t run = 0;
     for (int d = 0; d <= kdmax; ++d) {
-      const int32_t c = hist[d];
       A.kdoff[off + d] = run;
-      hist[d] = run;
-      run += c;
+      for (int b = 0; b < kBands; ++b) {
+        const int32_t c = hist[d * kBands + b];
+        hist[d * kBands + b] = run;
+        run += c;
+      }
     }
     A.kdoff[off + kdmax + 1] = run;
     int32_t leafpos = 0;
     for (int k = 0; k < K; ++k) {
       const int32_t i = A.kr[off + k];
-      const int32_t pos = (orl[i] == i) ? leafpos++ : K1 + hist[kdn[i]]++;
+      const int32_t pos =
+          (orl[i] == i) ? leafpos++ : K1 + hist[kdn[i] * kBands + band(orl[i] - i + 1)]++;
       A.skr[off + pos] = i;
       A.skd[off + pos] = kdn[i];
       A.ssize[off + pos] = orl[i] - i + 1;

@@ -26,21 +26,25 @@ ERR_CANCELLED = 10
 
 CLASS_AUTO = -1
 CLASS_LANE = 0
-NUM_CLASSES = 33
+NUM_CLASSES = 34
+CLASS_TINY = 33
 XL_BW = (4096, 2048, 1024, 512, 256, 128)  # column-block widths of classes 27..32
 
 
 def class_name(cls):
     """'leaf' (closed form for tables with a leaf keyroot) or
     'rowscan_w{W}_c{C}' (W waves, C columns per lane) or 'xl_bw{B}' (column
-    blocks of B columns, one workgroup each) or 'too_large'."""
+    blocks of B columns, one workgroup each) or 'tiny_s16' (tables of at most
+    16 columns, four to a wave) or 'too_large'."""
     if cls == 0:
         return "leaf"
     if 1 <= cls <= 25:
         wi, ci = divmod(cls - 1, 5)
         return f"rowscan_w{1 << wi}_c{1 << ci}"
-    if 27 <= cls < NUM_CLASSES:
+    if 27 <= cls < 27 + len(XL_BW):
         return f"xl_bw{XL_BW[cls - 27]}"
+    if cls == CLASS_TINY:
+        return "tiny_s16"
     return "too_large"
 
 GEN_RRT, GEN_DEEP, GEN_SHALLOW = 0, 1, 2
