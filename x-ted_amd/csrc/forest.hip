@@ -83,7 +83,10 @@ __device__ __forceinline__ int wave_shr1(int v, int fill) {
 // ---------------------------------------------------------------------------
 constexpr int kLeafTile = 64;
 __global__ __launch_bounds__(256) void leaf_kernel(LeafArgs A, int64_t ntiles) {
-  __shared__ int32_t tile[kLeafTile][kLeafTile + 1];
+  // Row kLeafTile of the tile takes the values of lines past the tile's
+  // count, which are computed (so a wave's lines have no branches between
+  // them and their scans interleave) but never stored.
+  __shared__ int32_t tile[kLeafTile + 1][kLeafTile + 1];
   __shared__ int32_t lines[kLeafTile];
   const int64_t tix = blockIdx.x;
   if (tix >= ntiles) return;
@@ -144,16 +147,15 @@ __global__ __launch_bounds__(256) void leaf_kernel(LeafArgs A, int64_t ntiles) {
 #pragma unroll
     for (int m = 0; m < kPer; ++m) {
       const int k = wave + 4 * m;  // lines interleaved over the waves
-      if (k < nl) {                // uniform
-        const int32_t off = lines[k];
-        const int32_t lam = __builtin_amdgcn_readlane(laml, off);
-        const int32_t incl = wave_incl_min(c >= 0 && lab == lam ? c : INT_MAX);
-        const int32_t nxt = min(carry[m], incl);
-        carry[m] = min(carry[m], __builtin_amdgcn_readlane(incl, 63));
-        const int32_t d = size - (int32_t)(nxt <= oc);
-        if (c >= 0) rows[(int64_t)(v0 + off) * T.n + c] = d;
-        tile[off][lane] = d;
-      }
+      const bool real = k < nl;    // uniform
+      const int32_t off = real ? lines[k] : kLeafTile;
+      const int32_t lam = __builtin_amdgcn_readlane(laml, real ? off : 0);
+      const int32_t incl = wave_incl_min(c >= 0 && lab == lam ? c : INT_MAX);
+      const int32_t nxt = min(carry[m], incl);
+      carry[m] = min(carry[m], __builtin_amdgcn_readlane(incl, 63));
+      const int32_t d = size - (int32_t)(nxt <= oc);
+      if (real && c >= 0) rows[(int64_t)(v0 + off) * T.n + c] = d;
+      tile[off][lane] = d;
     }
     const unsigned long long skip = __ballot(c < 0 || c_line);
     __syncthreads();
