@@ -80,6 +80,27 @@ int xted_ted(xted_context* ctx, const int32_t* x_parent, const int32_t* x_label,
              const int32_t* y_parent, const int32_t* y_label, int32_t n, int32_t* td,
              int32_t* dist, xted_stats* stats);
 
+/* Optimal edit mapping of x (m nodes) and y (n nodes): the tree-distance
+ * table is computed on the device as by xted_ted, then an optimal mapping is
+ * traced back on the host through the forest tables of the tree pair and of
+ * the subtree pairs it matches as units. map_x[k], map_y[k] (capacity
+ * min(m, n) each) receive the *nmap matched node pairs in increasing order of
+ * both; unmatched x nodes are deleted, unmatched y nodes inserted, matched
+ * pairs with different labels renamed, and the cost of that script is
+ * *dist. No counterpart in the reference, which returns distances only;
+ * m * n at most 2^26 (XTED_ERR_TOO_LARGE beyond). */
+int xted_edit_mapping(xted_context* ctx, const int32_t* x_parent, const int32_t* x_label,
+                      int32_t m, const int32_t* y_parent, const int32_t* y_label, int32_t n,
+                      int32_t* map_x, int32_t* map_y, int32_t* nmap, int32_t* dist);
+/* The host half of xted_edit_mapping, for a tree-distance table the caller
+ * already holds (xted_ted or xted_batch output): td is m x n, row-major.
+ * *cost receives the mapping's cost (td[0] for a correct table). Runs on the
+ * host only. */
+int xted_mapping_from_td(const int32_t* x_parent, const int32_t* x_label, int32_t m,
+                         const int32_t* y_parent, const int32_t* y_label, int32_t n,
+                         const int32_t* td, int32_t* map_x, int32_t* map_y, int32_t* nmap,
+                         int64_t* cost);
+
 /* ---- structural arrays --------------------------------------------------------
  * Device-computed outermost-right leaves (orl, n ints), keyroots in
  * descending preorder (kr, up to n ints, count in *K) and keyroot depth per

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "mapping.hpp"
 #include "server.hpp"
 #include "trees.hpp"
 
@@ -131,6 +132,72 @@ int xted_ted(xted_context* ctx, const int32_t* x_parent, const int32_t* x_label,
     st = c.run_batch(1, &ids[0], &ids[1], dist, td ? tds : nullptr, &bs);
     to_c(bs, stats);
     return st;
+  });
+}
+
+int xted_edit_mapping(xted_context* ctx, const int32_t* x_parent, const int32_t* x_label,
+                      int32_t m, const int32_t* y_parent, const int32_t* y_label, int32_t n,
+                      int32_t* map_x, int32_t* map_y, int32_t* nmap, int32_t* dist) {
+  return guarded([&]() -> int {
+    if (!ctx || !nmap || !dist || m < 0 || n < 0 || (m > 0 && n > 0 && (!map_x || !map_y))) {
+      set_error("xted_edit_mapping: invalid argument");
+      return ERR_INVALID_ARG;
+    }
+    *nmap = 0;
+    if (m == 0 || n == 0) {  // X-TED_GPU/TED.cu:119-124
+      *dist = m + n;
+      return OK;
+    }
+    if ((int64_t)m * n > kMaxMappingCells) {
+      set_error("xted_edit_mapping: m*n above 2^26");
+      return ERR_TOO_LARGE;
+    }
+    std::vector<int32_t> td((size_t)m * n);
+    int st = xted_ted(ctx, x_parent, x_label, m, y_parent, y_label, n, td.data(), dist, nullptr);
+    if (st != OK) return st;
+    std::vector<std::pair<int32_t, int32_t>> map;
+    const int64_t cost = edit_mapping(x_parent, x_label, m, y_parent, y_label, n, td.data(), map);
+    if (cost != *dist) {
+      set_error("xted_edit_mapping: traced mapping costs " + std::to_string(cost) +
+                ", the table says " + std::to_string(*dist));
+      return ERR_DEVICE;
+    }
+    for (size_t k = 0; k < map.size(); ++k) {
+      map_x[k] = map[k].first;
+      map_y[k] = map[k].second;
+    }
+    *nmap = (int32_t)map.size();
+    return OK;
+  });
+}
+
+int xted_mapping_from_td(const int32_t* x_parent, const int32_t* x_label, int32_t m,
+                         const int32_t* y_parent, const int32_t* y_label, int32_t n,
+                         const int32_t* td, int32_t* map_x, int32_t* map_y, int32_t* nmap,
+                         int64_t* cost) {
+  return guarded([&]() -> int {
+    if (!nmap || !cost || m < 0 || n < 0 ||
+        (m > 0 && n > 0 && (!td || !map_x || !map_y || !x_parent || !x_label || !y_parent ||
+                            !y_label))) {
+      set_error("xted_mapping_from_td: invalid argument");
+      return ERR_INVALID_ARG;
+    }
+    if ((int64_t)m * n > kMaxMappingCells) {
+      set_error("xted_mapping_from_td: m*n above 2^26");
+      return ERR_TOO_LARGE;
+    }
+    int st;
+    if ((m > 0 && (st = validate_preorder(x_parent, m)) != OK) ||
+        (n > 0 && (st = validate_preorder(y_parent, n)) != OK))
+      return st;
+    std::vector<std::pair<int32_t, int32_t>> map;
+    *cost = edit_mapping(x_parent, x_label, m, y_parent, y_label, n, td, map);
+    for (size_t k = 0; k < map.size(); ++k) {
+      map_x[k] = map[k].first;
+      map_y[k] = map[k].second;
+    }
+    *nmap = (int32_t)map.size();
+    return OK;
   });
 }
 

@@ -61,6 +61,11 @@ _SIGNATURES = {
                                 ctypes.c_int32, _I32P, _I32P, ctypes.c_void_p]),
     "xted_tree_structure": (ctypes.c_int, [ctypes.c_void_p, _I32P, ctypes.c_int32, _I32P, _I32P,
                                            _I32P, _I32P]),
+    "xted_edit_mapping": (ctypes.c_int, [ctypes.c_void_p, _I32P, _I32P, ctypes.c_int32, _I32P,
+                                         _I32P, ctypes.c_int32, _I32P, _I32P, _I32P, _I32P]),
+    "xted_mapping_from_td": (ctypes.c_int, [_I32P, _I32P, ctypes.c_int32, _I32P, _I32P,
+                                            ctypes.c_int32, _I32P, _I32P, _I32P, _I32P,
+                                            ctypes.POINTER(ctypes.c_int64)]),
     "xted_pool_add": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32P, _I32P, _I32P, _I32P]),
     "xted_pool_clear": (ctypes.c_int, [ctypes.c_void_p]),
     "xted_pool_size": (ctypes.c_int, [ctypes.c_void_p, _I32P]),
@@ -159,6 +164,22 @@ def derive_seed(config, index):
     return int(lib().xted_derive_seed(config, index))
 
 
+def mapping_from_td(x, y, td):
+    """Host-only backtrace: an optimal edit mapping of x and y from their
+    m x n tree-distance table td. Returns (cost, pairs)."""
+    xp, xl = _tree(*x)
+    yp, yl = _tree(*y)
+    m, n = len(xp), len(yp)
+    td = np.ascontiguousarray(td, dtype=np.int32)
+    cap = max(1, min(m, n))
+    mx, my = np.empty(cap, np.int32), np.empty(cap, np.int32)
+    k, cost = ctypes.c_int32(), ctypes.c_int64()
+    _check(lib().xted_mapping_from_td(_p(xp), _p(xl), m, _p(yp), _p(yl), n,
+                                      _p(td) if m and n else None, _p(mx), _p(my),
+                                      ctypes.byref(k), ctypes.byref(cost)))
+    return cost.value, np.stack([mx[:k.value], my[:k.value]], axis=1)
+
+
 def generate(kind, n, alphabet, seed, depth_min=0, depth_max=0):
     """Seeded preorder tree: (parent, label) int32 arrays."""
     parent = np.empty(n, np.int32)
@@ -220,6 +241,19 @@ class Context:
                               _p(td) if td is not None else None, ctypes.byref(dist),
                               ctypes.byref(st)))
         return dist.value, td, st
+
+    def edit_mapping(self, x, y):
+        """An optimal edit mapping of x and y: (dist, pairs) with pairs an
+        (k, 2) array of matched (x node, y node), increasing in both."""
+        xp, xl = _tree(*x)
+        yp, yl = _tree(*y)
+        m, n = len(xp), len(yp)
+        cap = max(1, min(m, n))
+        mx, my = np.empty(cap, np.int32), np.empty(cap, np.int32)
+        k, dist = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().xted_edit_mapping(self._h, _p(xp), _p(xl), m, _p(yp), _p(yl), n, _p(mx),
+                                       _p(my), ctypes.byref(k), ctypes.byref(dist)))
+        return dist.value, np.stack([mx[:k.value], my[:k.value]], axis=1)
 
     def tree_structure(self, parent):
         """Device-computed (orl, keyroots descending, keyroot depths)."""
