@@ -27,6 +27,22 @@ int oracle_keyroot_depth(const int32_t* orl, const int32_t* kr, int32_t K, int32
 int oracle_ted(const int32_t* x_parent, const int32_t* x_label, int32_t m,
                const int32_t* y_parent, const int32_t* y_label, int32_t n, int32_t threads,
                int32_t* td, double* compute_ms);
+/* Non-unit costs (no counterpart in the reference, which hard-codes unit
+ * costs, X-TED_CPU/TED.cpp:111,115,122): labels in [0, alphabet), del[a] and
+ * ins[a] for deleting / inserting a node labelled a, ren[a*alphabet + b] for
+ * relabelling a to b. A NULL costs pointer means unit costs. */
+typedef struct {
+  int32_t alphabet;
+  const int32_t* del;
+  const int32_t* ins;
+  const int32_t* ren;
+} oracle_costs;
+/* oracle_ted under a cost model; the empty-tree answers become the sum of
+ * the other tree's insertion (deletion) costs. */
+int oracle_ted_costs(const int32_t* x_parent, const int32_t* x_label, int32_t m,
+                     const int32_t* y_parent, const int32_t* y_label, int32_t n,
+                     const oracle_costs* costs, int32_t threads, int32_t* td,
+                     double* compute_ms);
 /* Level of every table (k, l): depth_x[k] + depth_y[l], K*L entries. */
 int oracle_levels(const int32_t* x_parent, int32_t m, const int32_t* y_parent, int32_t n,
                   int32_t* levels, int32_t* K, int32_t* L);
@@ -35,6 +51,9 @@ int oracle_levels(const int32_t* x_parent, int32_t m, const int32_t* y_parent, i
  * reference's preorder formulation. Same output convention. */
 int zs_textbook(const int32_t* x_parent, const int32_t* x_label, int32_t m,
                 const int32_t* y_parent, const int32_t* y_label, int32_t n, int32_t* td);
+int zs_textbook_costs(const int32_t* x_parent, const int32_t* x_label, int32_t m,
+                      const int32_t* y_parent, const int32_t* y_label, int32_t n,
+                      const oracle_costs* costs, int32_t* td);
 
 #ifdef __cplusplus
 }

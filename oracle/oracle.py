@@ -21,6 +21,22 @@ _lib = None
 _ref = None
 
 
+class _Costs(ctypes.Structure):
+    _fields_ = [("alphabet", ctypes.c_int32), ("dele", _I32P), ("ins", _I32P), ("ren", _I32P)]
+
+
+def _costs(costs):
+    """costs = (dele[A], ins[A], ren[A, A]) int32 arrays, or None for unit
+    costs. Returns (struct pointer or None, arrays kept alive)."""
+    if costs is None:
+        return None, ()
+    dele, ins, ren = (np.ascontiguousarray(a, dtype=np.int32) for a in costs)
+    a = len(dele)
+    assert len(ins) == a and ren.shape == (a, a)
+    c = _Costs(a, dele.ctypes.data_as(_I32P), ins.ctypes.data_as(_I32P), ren.ctypes.data_as(_I32P))
+    return ctypes.pointer(c), (dele, ins, ren, c)
+
+
 def _p(a):
     return a.ctypes.data_as(_I32P)
 
@@ -35,6 +51,11 @@ def _load():
                                     ctypes.c_int32, _I32P, ctypes.POINTER(ctypes.c_double)]
         _lib.zs_textbook.argtypes = [_I32P, _I32P, ctypes.c_int32, _I32P, _I32P, ctypes.c_int32,
                                      _I32P]
+        _lib.oracle_ted_costs.argtypes = [_I32P, _I32P, ctypes.c_int32, _I32P, _I32P,
+                                          ctypes.c_int32, ctypes.POINTER(_Costs), ctypes.c_int32,
+                                          _I32P, ctypes.POINTER(ctypes.c_double)]
+        _lib.zs_textbook_costs.argtypes = [_I32P, _I32P, ctypes.c_int32, _I32P, _I32P,
+                                           ctypes.c_int32, ctypes.POINTER(_Costs), _I32P]
         _lib.oracle_orl.argtypes = [_I32P, ctypes.c_int32, _I32P]
         _lib.oracle_keyroots.argtypes = [_I32P, ctypes.c_int32, _I32P, _I32P]
         _lib.oracle_keyroot_depth.argtypes = [_I32P, _I32P, ctypes.c_int32, _I32P]
@@ -77,6 +98,32 @@ def ted(x, y, threads=1, want_td=True):
     if d < 0:
         raise ValueError("oracle rejected the input")
     return d, td, ms.value
+
+
+def ted_costs(x, y, costs, threads=1, want_td=True):
+    """oracle_ted under a cost model (dele, ins, ren arrays; None = unit):
+    (dist, td)."""
+    xp, xl = map(_arr, x)
+    yp, yl = map(_arr, y)
+    m, n = len(xp), len(yp)
+    td = np.empty((m, n), np.int32) if (want_td and m and n) else None
+    cp, keep = _costs(costs)
+    d = _load().oracle_ted_costs(_p(xp), _p(xl), m, _p(yp), _p(yl), n, cp, threads,
+                                 _p(td) if td is not None else None, None)
+    if d < 0:
+        raise ValueError("oracle rejected the input")
+    return d, td
+
+
+def textbook_costs(x, y, costs):
+    xp, xl = map(_arr, x)
+    yp, yl = map(_arr, y)
+    m, n = len(xp), len(yp)
+    td = np.empty((m, n), np.int32) if (m and n) else None
+    cp, keep = _costs(costs)
+    d = _load().zs_textbook_costs(_p(xp), _p(xl), m, _p(yp), _p(yl), n, cp,
+                                  _p(td) if td is not None else None)
+    return d, td
 
 
 def textbook(x, y):
@@ -150,12 +197,20 @@ def ref_key_roots(orl_arr):
 
 # ---- pure-Python checks for tiny inputs -----------------------------------------
 
-def levenshtein(a, b):
-    prev = list(range(len(b) + 1))
-    for i, ca in enumerate(a, 1):
-        cur = [i]
+def levenshtein(a, b, costs=None):
+    """Edit distance of two sequences; with costs = (dele, ins, ren) the
+    weighted one (deleting a costs dele[a], inserting b ins[b], replacing a
+    by b ren[a][b])."""
+    dl = (lambda c: 1) if costs is None else (lambda c: int(costs[0][c]))
+    il = (lambda c: 1) if costs is None else (lambda c: int(costs[1][c]))
+    rl = (lambda p, q: int(p != q)) if costs is None else (lambda p, q: int(costs[2][p][q]))
+    prev = [0]
+    for cb in b:
+        prev.append(prev[-1] + il(cb))
+    for ca in a:
+        cur = [prev[0] + dl(ca)]
         for j, cb in enumerate(b, 1):
-            cur.append(min(prev[j] + 1, cur[j - 1] + 1, prev[j - 1] + (ca != cb)))
+            cur.append(min(prev[j] + dl(ca), cur[j - 1] + il(cb), prev[j - 1] + rl(ca, cb)))
         prev = cur
     return prev[-1]
 
@@ -168,15 +223,20 @@ def _ancestors(parent):
     return anc
 
 
-def brute_force_ted(x, y):
+def brute_force_ted(x, y, costs=None):
     """Minimum over all valid edit mappings (Tai 1979): a one-to-one partial
     map M that preserves preorder order and ancestry; cost = renames in M +
-    deletions + insertions. Exponential; use for trees of <= 6 nodes."""
+    deletions + insertions (unit, or the (dele, ins, ren) cost tables).
+    Exponential; use for trees of <= 6 nodes."""
     xp, xl = [list(map(int, a)) for a in x]
     yp, yl = [list(map(int, a)) for a in y]
     m, n = len(xp), len(yp)
+    dl = [1] * m if costs is None else [int(costs[0][a]) for a in xl]
+    il = [1] * n if costs is None else [int(costs[1][b]) for b in yl]
+    ren = (lambda i, j: int(xl[i] != yl[j])) if costs is None else \
+        (lambda i, j: int(costs[2][xl[i]][yl[j]]))
     ax, ay = _ancestors(xp), _ancestors(yp)
-    best = m + n
+    best = sum(dl) + sum(il)
     for k in range(0, min(m, n) + 1):
         for xs in itertools.combinations(range(m), k):
             for ys in itertools.combinations(range(n), k):
@@ -190,6 +250,7 @@ def brute_force_ted(x, y):
                         break
                 if not ok:
                     continue
-                cost = sum(xl[i] != yl[j] for i, j in zip(xs, ys)) + (m - k) + (n - k)
+                cost = (sum(ren(i, j) for i, j in zip(xs, ys)) + sum(dl) - sum(dl[i] for i in xs)
+                        + sum(il) - sum(il[j] for j in ys))
                 best = min(best, cost)
     return best

@@ -97,6 +97,12 @@ struct Problem {
   const int32_t* yl;
   std::vector<int> xo, yo, xk, yk;
   int32_t* td;  // m*n
+  const oracle_costs* costs;  // NULL: unit costs
+  int del(int i) const { return costs ? costs->del[xl[i]] : 1; }
+  int ins(int j) const { return costs ? costs->ins[yl[j]] : 1; }
+  int ren(int i, int j) const {
+    return costs ? costs->ren[(size_t)xl[i] * costs->alphabet + yl[j]] : (xl[i] != yl[j] ? 1 : 0);
+  }
 };
 
 // X-TED_CPU/parallel_computing.cpp:4-45: one keyroot-pair table, forest F is
@@ -106,18 +112,17 @@ void compute_table(const Problem& P, int k, int l, int32_t* F) {
   const int i0 = P.xk[k], j0 = P.yk[l];
   const int imax = P.xo[i0] + 1, jmax = P.yo[j0] + 1;
   F[imax * W + jmax] = 0;
-  for (int i = imax - 1; i >= i0; --i) F[i * W + jmax] = 1 + F[(i + 1) * W + jmax];
-  for (int j = jmax - 1; j >= j0; --j) F[imax * W + j] = 1 + F[imax * W + j + 1];
+  for (int i = imax - 1; i >= i0; --i) F[i * W + jmax] = P.del(i) + F[(i + 1) * W + jmax];
+  for (int j = jmax - 1; j >= j0; --j) F[imax * W + j] = P.ins(j) + F[imax * W + j + 1];
   for (int i = imax - 1; i >= i0; --i) {
     for (int j = jmax - 1; j >= j0; --j) {
       int v;
+      const int dv = P.del(i) + F[(i + 1) * W + j], iv = P.ins(j) + F[i * W + j + 1];
       if (P.xo[i] == P.xo[i0] && P.yo[j] == P.yo[j0]) {
-        const int cost = P.xl[i] != P.yl[j] ? 1 : 0;
-        v = std::min({cost + F[(i + 1) * W + j + 1], 1 + F[(i + 1) * W + j], 1 + F[i * W + j + 1]});
+        v = std::min({P.ren(i, j) + F[(i + 1) * W + j + 1], dv, iv});
         P.td[(size_t)i * P.n + j] = v;
       } else {
-        v = std::min({P.td[(size_t)i * P.n + j] + F[(P.xo[i] + 1) * W + P.yo[j] + 1],
-                      1 + F[(i + 1) * W + j], 1 + F[i * W + j + 1]});
+        v = std::min({P.td[(size_t)i * P.n + j] + F[(P.xo[i] + 1) * W + P.yo[j] + 1], dv, iv});
       }
       F[i * W + j] = v;
     }
@@ -170,14 +175,28 @@ int oracle_levels(const int32_t* x_parent, int32_t m, const int32_t* y_parent, i
 int oracle_ted(const int32_t* x_parent, const int32_t* x_label, int32_t m,
                const int32_t* y_parent, const int32_t* y_label, int32_t n, int32_t threads,
                int32_t* td, double* compute_ms) {
+  return oracle_ted_costs(x_parent, x_label, m, y_parent, y_label, n, nullptr, threads, td,
+                          compute_ms);
+}
+
+int oracle_ted_costs(const int32_t* x_parent, const int32_t* x_label, int32_t m,
+                     const int32_t* y_parent, const int32_t* y_label, int32_t n,
+                     const oracle_costs* costs, int32_t threads, int32_t* td,
+                     double* compute_ms) {
   if (compute_ms) *compute_ms = 0.0;
   if (m < 0 || n < 0) return -1;
-  // X-TED_CPU/TED.cpp:181-186
-  if (m == 0) return n;
-  if (n == 0) return m;
+  // X-TED_CPU/TED.cpp:181-186 (with costs: every node of the other tree
+  // inserted, or deleted)
+  if (m == 0 || n == 0) {
+    int s = 0;
+    for (int j = 0; j < n; ++j) s += costs ? costs->ins[y_label[j]] : 1;
+    for (int i = 0; i < m; ++i) s += costs ? costs->del[x_label[i]] : 1;
+    return s;
+  }
   if (!valid_parent(x_parent, m) || !valid_parent(y_parent, n)) return -1;
   if (threads < 1) threads = 1;
   Problem P;
+  P.costs = costs;
   P.m = m;
   P.n = n;
   P.xl = x_label;

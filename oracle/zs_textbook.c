@@ -102,8 +102,26 @@ static void free_post(Post* P) {
 
 int zs_textbook(const int32_t* x_parent, const int32_t* x_label, int32_t m,
                 const int32_t* y_parent, const int32_t* y_label, int32_t n, int32_t* td) {
-  if (m == 0) return n;
-  if (n == 0) return m;
+  return zs_textbook_costs(x_parent, x_label, m, y_parent, y_label, n, NULL, td);
+}
+
+/* Cost of deleting / inserting the node at postorder position p, and of
+ * relabelling a onto b. */
+static int c_del(const oracle_costs* C, const Post* A, int p) { return C ? C->del[A->lab[p]] : 1; }
+static int c_ins(const oracle_costs* C, const Post* B, int p) { return C ? C->ins[B->lab[p]] : 1; }
+static int c_ren(const oracle_costs* C, const Post* A, int i, const Post* B, int j) {
+  return C ? C->ren[(size_t)A->lab[i] * C->alphabet + B->lab[j]] : (A->lab[i] != B->lab[j]);
+}
+
+int zs_textbook_costs(const int32_t* x_parent, const int32_t* x_label, int32_t m,
+                      const int32_t* y_parent, const int32_t* y_label, int32_t n,
+                      const oracle_costs* C, int32_t* td) {
+  if (m == 0 || n == 0) {
+    int s = 0;
+    for (int j = 0; j < n; ++j) s += C ? C->ins[y_label[j]] : 1;
+    for (int i = 0; i < m; ++i) s += C ? C->del[x_label[i]] : 1;
+    return s;
+  }
   Post A, B;
   build_post(x_parent, x_label, m, &A);
   build_post(y_parent, y_label, n, &B);
@@ -117,14 +135,16 @@ int zs_textbook(const int32_t* x_parent, const int32_t* x_label, int32_t m,
       const int li = A.lml[i], lj = B.lml[j];
       /* fd rows li-1..i, cols lj-1..j (absolute postorder positions) */
       fd[(li - 1) * W + (lj - 1)] = 0;
-      for (int i1 = li; i1 <= i; ++i1) fd[i1 * W + (lj - 1)] = fd[(i1 - 1) * W + (lj - 1)] + 1;
-      for (int j1 = lj; j1 <= j; ++j1) fd[(li - 1) * W + j1] = fd[(li - 1) * W + j1 - 1] + 1;
+      for (int i1 = li; i1 <= i; ++i1)
+        fd[i1 * W + (lj - 1)] = fd[(i1 - 1) * W + (lj - 1)] + c_del(C, &A, i1);
+      for (int j1 = lj; j1 <= j; ++j1)
+        fd[(li - 1) * W + j1] = fd[(li - 1) * W + j1 - 1] + c_ins(C, &B, j1);
       for (int i1 = li; i1 <= i; ++i1) {
         for (int j1 = lj; j1 <= j; ++j1) {
-          const int del = fd[(i1 - 1) * W + j1] + 1;
-          const int ins = fd[i1 * W + j1 - 1] + 1;
+          const int del = fd[(i1 - 1) * W + j1] + c_del(C, &A, i1);
+          const int ins = fd[i1 * W + j1 - 1] + c_ins(C, &B, j1);
           if (A.lml[i1] == li && B.lml[j1] == lj) {
-            const int ren = fd[(i1 - 1) * W + j1 - 1] + (A.lab[i1] != B.lab[j1]);
+            const int ren = fd[(i1 - 1) * W + j1 - 1] + c_ren(C, &A, i1, &B, j1);
             const int v = min3i(del, ins, ren);
             fd[i1 * W + j1] = v;
             T[i1 * W + j1] = v;
