@@ -88,7 +88,8 @@ int xted_ted(xted_context* ctx, const int32_t* x_parent, const int32_t* x_label,
  * both; unmatched x nodes are deleted, unmatched y nodes inserted, matched
  * pairs with different labels renamed, and the cost of that script is
  * *dist. No counterpart in the reference, which returns distances only;
- * m * n at most 2^26 (XTED_ERR_TOO_LARGE beyond). */
+ * m * n at most 2^26 (XTED_ERR_TOO_LARGE beyond). Unit costs only: a
+ * context under a cost model (xted_set_costs) returns XTED_ERR_INVALID_ARG. */
 int xted_edit_mapping(xted_context* ctx, const int32_t* x_parent, const int32_t* x_label,
                       int32_t m, const int32_t* y_parent, const int32_t* y_label, int32_t n,
                       int32_t* map_x, int32_t* map_y, int32_t* nmap, int32_t* dist);
@@ -165,6 +166,22 @@ int xted_set_batch_groups(xted_context* ctx, int32_t groups, int32_t split_min);
  * one by one regardless. The reference launches every kernel directly
  * (X-TED_GPU/parallel_X_TED.cu:429-443). Results do not depend on it. */
 int xted_set_graphs(xted_context* ctx, int32_t on);
+/* Non-unit edit costs (the reference hard-codes unit costs,
+ * X-TED_CPU/TED.cpp:111,115,122): labels in [0, alphabet) (alphabet at most
+ * 4096), del[a] / ins[a] for deleting / inserting a node labelled a and
+ * ren[a*alphabet + b] for relabelling a (an x label) to b (a y label), each
+ * in [0, 32767]. Applies to every later xted_ted, xted_batch and server
+ * query of the context; every table then takes the row-scan forms (the
+ * closed-form leaf tables and the tiny form assume unit costs), and a pair
+ * whose forest values can exceed 65535 (deleting all of x plus inserting
+ * all of y) is rejected with XTED_ERR_TOO_LARGE. xted_ted of an empty tree
+ * returns the cost of inserting (deleting) every node of the other.
+ * del == NULL restores unit costs. Under a cost model xted_pool_add rejects
+ * a tree with a label outside the alphabet; a tree added before the model
+ * stays in the pool, and a query that uses one fails with
+ * XTED_ERR_INVALID_ARG. */
+int xted_set_costs(xted_context* ctx, int32_t alphabet, const int32_t* del, const int32_t* ins,
+                   const int32_t* ren);
 
 /* ---- per-class device timing ---------------------------------------------------
  * With profiling on, xted_ted / xted_batch bracket every forest-DP launch

@@ -1,0 +1,231 @@
+"""GPU parity under non-unit edit costs (per-label delete / insert tables and
+a rename matrix; the reference hard-codes unit costs,
+X-TED_CPU/TED.cpp:111,115,122): the device's cost mode against the oracle's
+cost restatement, bit-exact on distances and full tree-distance tables, over
+random cost models (zero costs, asymmetric insert / delete, rename above
+delete + insert), both table orientations, every row-scan class and every
+column-blocked form forced, batches, and values near the 16-bit slot limit."""
+import numpy as np
+import pytest
+
+import oracle
+import xted
+
+pytestmark = pytest.mark.gpu
+
+
+def gen(kind, n, alphabet, cfg, idx, band=(0, 0)):
+    return xted.generate(kind, n, alphabet, xted.derive_seed(cfg, idx), *band)
+
+
+def gen_any(kind, n, alphabet, cfg, idx):
+    """Deep trees get a depth band the node count can reach."""
+    if kind == xted.GEN_DEEP and n < 8:
+        kind = xted.GEN_RRT
+    band = (n // 8, n // 3) if kind == xted.GEN_DEEP else (0, 0)
+    return gen(kind, n, alphabet, cfg, idx, band)
+
+
+def rand_costs(rng, alphabet, hi=9, zeros=False):
+    lo = 0 if zeros else 1
+    dele = rng.integers(lo, hi, alphabet).astype(np.int32)
+    ins = rng.integers(lo, hi, alphabet).astype(np.int32)
+    ren = rng.integers(0, 2 * hi, (alphabet, alphabet)).astype(np.int32)
+    np.fill_diagonal(ren, 0)
+    return dele, ins, ren
+
+
+def unit_costs(alphabet):
+    ren = np.ones((alphabet, alphabet), np.int32)
+    np.fill_diagonal(ren, 0)
+    return np.ones(alphabet, np.int32), np.ones(alphabet, np.int32), ren
+
+
+@pytest.fixture
+def cost_ctx(gpu_ctx):
+    yield gpu_ctx
+    gpu_ctx.set_costs(None)
+    gpu_ctx.force_class(xted.CLASS_AUTO, lane_enable=True)
+
+
+def assert_same(ctx, x, y, costs, threads=8):
+    d, td, _ = ctx.ted(x, y)
+    od, otd = oracle.ted_costs(x, y, costs, threads=threads)
+    assert d == od
+    mism = np.argwhere(td != otd)
+    assert len(mism) == 0, f"{len(mism)} TD cells differ, first {mism[:5].tolist()}"
+    return d
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_cost_models(cost_ctx, seed):
+    rng = np.random.default_rng(100 + seed)
+    alphabet = int(rng.choice([3, 8, 40]))
+    costs = rand_costs(rng, alphabet, hi=int(rng.choice([3, 9, 60])), zeros=seed % 2 == 1)
+    cost_ctx.set_costs(costs)
+    for i in range(4):
+        kx, ky = int(rng.integers(0, 3)), int(rng.integers(0, 3))
+        n1, n2 = int(rng.integers(1, 260)), int(rng.integers(1, 260))
+        x = gen_any(kx, n1, alphabet, 200 + seed, 2 * i)
+        y = gen_any(ky, n2, alphabet, 200 + seed, 2 * i + 1)
+        assert_same(cost_ctx, x, y, costs)
+        assert_same(cost_ctx, y, x, costs)
+
+
+def test_unit_tables_match_unit_mode(cost_ctx):
+    """A cost model of unit tables takes the cost kernels and must reproduce
+    unit mode, which takes the leaf and tiny forms as well."""
+    x = gen(xted.GEN_DEEP, 1000, 30, 210, 0, (60, 120))
+    y = gen(xted.GEN_SHALLOW, 900, 30, 210, 1)
+    d0, t0, _ = cost_ctx.ted(x, y)
+    cost_ctx.set_costs(unit_costs(30))
+    d1, t1, _ = cost_ctx.ted(x, y)
+    assert d0 == d1 and np.array_equal(t0, t1)
+
+
+@pytest.mark.parametrize("kind", [xted.GEN_RRT, xted.GEN_DEEP, xted.GEN_SHALLOW])
+def test_thousand_node_pairs(cost_ctx, kind):
+    rng = np.random.default_rng(220 + kind)
+    costs = rand_costs(rng, 50)
+    cost_ctx.set_costs(costs)
+    band = (60, 120) if kind == xted.GEN_DEEP else (0, 0)
+    x = gen(kind, 1000, 50, 220, 2 * kind, band)
+    y = gen(kind, 700, 50, 220, 2 * kind + 1, band)
+    assert_same(cost_ctx, x, y, costs, threads=16)
+    assert_same(cost_ctx, y, x, costs, threads=16)
+
+
+@pytest.mark.parametrize("cls", list(range(1, 26)))
+def test_every_row_class_forced(cost_ctx, cls):
+    rng = np.random.default_rng(300 + cls)
+    costs = rand_costs(rng, 6, zeros=cls % 3 == 0)
+    cost_ctx.set_costs(costs)
+    cost_ctx.force_class(cls, lane_enable=False)
+    for i, (kx, ky, n1, n2) in enumerate([(0, 0, 90, 70), (1, 2, 120, 150), (2, 1, 80, 130)]):
+        x = gen(kx, n1, 6, 300 + cls, 2 * i)
+        y = gen(ky, n2, 6, 300 + cls, 2 * i + 1)
+        assert_same(cost_ctx, x, y, costs)
+
+
+@pytest.mark.parametrize("cls", [27, 28, 29, 30, 31, 32])
+def test_every_column_blocked_form_forced(cost_ctx, cls):
+    rng = np.random.default_rng(400 + cls)
+    costs = rand_costs(rng, 12)
+    cost_ctx.set_costs(costs)
+    cost_ctx.force_class(cls)
+    for i, (kx, ky, n1, n2) in enumerate([(0, 1, 300, 420), (1, 2, 900, 1200)]):
+        x = gen(kx, n1, 12, 400 + cls, 2 * i)
+        y = gen(ky, n2, 12, 400 + cls, 2 * i + 1)
+        assert_same(cost_ctx, x, y, costs, threads=16)
+
+
+def test_tiny_class_falls_back_to_row_scan(cost_ctx):
+    """Forcing the tiny form (unit costs only) under a cost model leaves the
+    choice to the row-scan rule."""
+    rng = np.random.default_rng(5)
+    costs = rand_costs(rng, 5)
+    cost_ctx.set_costs(costs)
+    cost_ctx.force_class(xted.CLASS_TINY, lane_enable=False)
+    x = gen(xted.GEN_RRT, 60, 5, 500, 0)
+    y = gen(xted.GEN_SHALLOW, 50, 5, 500, 1)
+    assert_same(cost_ctx, x, y, costs)
+
+
+def test_batch_against_oracle(cost_ctx):
+    rng = np.random.default_rng(7)
+    costs = rand_costs(rng, 20)
+    trees = [gen_any(int(rng.integers(0, 3)), int(rng.integers(1, 500)), 20, 600, i)
+             for i in range(30)]
+    cost_ctx.pool_clear()
+    ids = cost_ctx.pool_add(trees)
+    cost_ctx.set_costs(costs)
+    xs = rng.integers(0, 30, 40).astype(np.int32)
+    ys = rng.integers(0, 30, 40).astype(np.int32)
+    shapes = [(len(trees[a][0]), len(trees[b][0])) for a, b in zip(xs, ys)]
+    dist, tds, _ = cost_ctx.batch(ids[xs], ids[ys], td_shapes=shapes)
+    # trees added after the cost model is set get their prefixes too
+    more = [gen(xted.GEN_RRT, 300, 20, 601, i) for i in range(2)]
+    mid = cost_ctx.pool_add(more)
+    d2, t2, _ = cost_ctx.batch(mid[:1], mid[1:], td_shapes=[(300, 300)])
+    for k, (a, b) in enumerate(zip(xs, ys)):
+        od, otd = oracle.ted_costs(trees[a], trees[b], costs, threads=4)
+        assert dist[k] == od and np.array_equal(tds[k], otd)
+    od, otd = oracle.ted_costs(more[0], more[1], costs, threads=4)
+    assert d2[0] == od and np.array_equal(t2[0], otd)
+
+
+def test_switching_cost_models(cost_ctx):
+    rng = np.random.default_rng(9)
+    x = gen(xted.GEN_RRT, 200, 10, 700, 0)
+    y = gen(xted.GEN_DEEP, 180, 10, 700, 1, (20, 40))
+    c1, c2 = rand_costs(rng, 10), rand_costs(rng, 10, hi=30)
+    cost_ctx.set_costs(c1)
+    assert_same(cost_ctx, x, y, c1)
+    cost_ctx.set_costs(c2)
+    assert_same(cost_ctx, x, y, c2)
+    cost_ctx.set_costs(None)
+    d, td, _ = cost_ctx.ted(x, y)
+    od, otd, _ = oracle.ted(x, y, threads=8)
+    assert d == od and np.array_equal(td, otd)
+
+
+def test_values_near_the_slot_limit(cost_ctx):
+    """Forest values up to 65535 fit the 16-bit row slots."""
+    alphabet = 4
+    dele = np.array([8000, 7000, 6000, 5000], np.int32)
+    ins = np.array([7500, 6500, 5500, 4500], np.int32)
+    ren = np.array([[0, 9000, 15000, 32767], [100, 0, 20000, 3],
+                    [13000, 1, 0, 25000], [32767, 40, 11111, 0]], np.int32)
+    costs = (dele, ins, ren)
+    cost_ctx.set_costs(costs)
+    rng = np.random.default_rng(11)
+    for i in range(12):
+        m, n = int(rng.integers(1, 6)), int(rng.integers(1, 6))
+        x = gen_any(int(rng.integers(0, 3)), m, alphabet, 800, 2 * i)
+        y = gen_any(int(rng.integers(0, 3)), n, alphabet, 800, 2 * i + 1)
+        if dele[x[1]].sum() + ins[y[1]].sum() > 65535:
+            with pytest.raises(xted.XtedError):
+                cost_ctx.ted(x, y)
+            continue
+        assert_same(cost_ctx, x, y, costs)
+
+
+def test_empty_trees_and_rejections(cost_ctx):
+    rng = np.random.default_rng(13)
+    costs = rand_costs(rng, 6)
+    cost_ctx.set_costs(costs)
+    x = gen(xted.GEN_RRT, 30, 6, 900, 0)
+    empty = (np.empty(0, np.int32), np.empty(0, np.int32))
+    assert cost_ctx.ted(x, empty)[0] == int(costs[0][x[1]].sum())
+    assert cost_ctx.ted(empty, x)[0] == int(costs[1][x[1]].sum())
+    bad = (x[0], np.full(30, 6, np.int32))
+    with pytest.raises(xted.XtedError):
+        cost_ctx.ted(x, bad)
+    with pytest.raises(xted.XtedError):
+        cost_ctx.edit_mapping(x, x)
+    with pytest.raises(xted.XtedError):
+        cost_ctx.set_costs((costs[0], costs[1], np.full((6, 6), 40000, np.int32)))
+    big = (np.full(6, 30000, np.int32), np.full(6, 30000, np.int32), costs[2])
+    cost_ctx.set_costs(big)
+    with pytest.raises(xted.XtedError):
+        cost_ctx.ted(x, x)
+    cost_ctx.set_costs(None)
+    assert cost_ctx.ted(x, bad)[0] >= 0
+
+
+def test_pool_trees_outside_the_alphabet(cost_ctx):
+    """Trees added before the cost model may carry labels it does not cover:
+    pairs of them are rejected, other pairs run."""
+    wide = [gen(xted.GEN_RRT, 80, 40, 950, i) for i in range(2)]
+    narrow = [gen(xted.GEN_SHALLOW, 70, 3, 951, i) for i in range(2)]
+    cost_ctx.pool_clear()
+    ids = cost_ctx.pool_add(wide + narrow)
+    costs = rand_costs(np.random.default_rng(17), 3)
+    cost_ctx.set_costs(costs)
+    with pytest.raises(xted.XtedError):
+        cost_ctx.batch(ids[:1], ids[2:3])
+    with pytest.raises(xted.XtedError):
+        cost_ctx.pool_add(wide[:1])
+    dist, tds, _ = cost_ctx.batch(ids[2:3], ids[3:4], td_shapes=[(70, 70)])
+    od, otd = oracle.ted_costs(narrow[0], narrow[1], costs, threads=4)
+    assert dist[0] == od and np.array_equal(tds[0], otd)

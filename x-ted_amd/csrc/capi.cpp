@@ -2,6 +2,7 @@
 #include "../../include/xted.h"
 
 #include <algorithm>
+#include <climits>
 #include <exception>
 #include <new>
 #include <vector>
@@ -102,20 +103,31 @@ int xted_ted(xted_context* ctx, const int32_t* x_parent, const int32_t* x_label,
       return ERR_INVALID_ARG;
     }
     if (stats) *stats = xted_stats{};
-    // X-TED_GPU/TED.cu:119-124
-    if (m == 0) {
-      *dist = n;
-      return OK;
-    }
-    if (n == 0) {
-      *dist = m;
+    Context& c = ctx->ctx;
+    // X-TED_GPU/TED.cu:119-124; under a cost model the distance to an empty
+    // tree is the cost of inserting (deleting) every node of the other.
+    if (m == 0 || n == 0) {
+      if (!c.cost_mode()) {
+        *dist = m == 0 ? n : m;
+        return OK;
+      }
+      const int32_t* lab = m == 0 ? y_label : x_label;
+      if (m + n > 0 && !lab) {
+        set_error("xted_ted: null tree array");
+        return ERR_INVALID_ARG;
+      }
+      const int64_t s = c.whole_tree_cost(lab, m + n, m > 0);
+      if (s < 0 || s > INT32_MAX) {
+        set_error(s < 0 ? "xted_ted: label outside the cost alphabet" : "xted_ted: cost overflow");
+        return s < 0 ? ERR_INVALID_ARG : ERR_TOO_LARGE;
+      }
+      *dist = (int32_t)s;
       return OK;
     }
     if (!x_parent || !x_label || !y_parent || !y_label) {
       set_error("xted_ted: null tree array");
       return ERR_INVALID_ARG;
     }
-    Context& c = ctx->ctx;
     int st = c.pool_clear();
     if (st != OK) return st;
     std::vector<int32_t> sizes{m, n};
@@ -144,6 +156,10 @@ int xted_edit_mapping(xted_context* ctx, const int32_t* x_parent, const int32_t*
       return ERR_INVALID_ARG;
     }
     *nmap = 0;
+    if (ctx->ctx.cost_mode()) {
+      set_error("xted_edit_mapping: unit costs only");
+      return ERR_INVALID_ARG;
+    }
     if (m == 0 || n == 0) {  // X-TED_GPU/TED.cu:119-124
       *dist = m + n;
       return OK;
@@ -316,6 +332,14 @@ int xted_set_batch_groups(xted_context* ctx, int32_t groups, int32_t split_min) 
   ctx->ctx.batch_groups = groups;
   ctx->ctx.batch_split_min = split_min;
   return OK;
+}
+
+int xted_set_costs(xted_context* ctx, int32_t alphabet, const int32_t* del, const int32_t* ins,
+                   const int32_t* ren) {
+  return guarded([&]() -> int {
+    if (!ctx) return ERR_INVALID_ARG;
+    return ctx->ctx.set_costs(alphabet, del, ins, ren);
+  });
 }
 
 int xted_set_graphs(xted_context* ctx, int32_t on) {

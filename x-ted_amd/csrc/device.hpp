@@ -60,6 +60,8 @@ struct PoolView {
   const int32_t* spre;   // prefix sums of ssize (K+1 entries)
   const int32_t* kdoff;
   const int32_t* rtop;   // keyroot at the top of node's right path
+  const int32_t* pdel;   // cost mode: deletion costs of nodes [0, i) (n+1 entries per tree)
+  const int32_t* pins;   // cost mode: insertion costs of nodes [0, i)
   const TreeMeta* trees;
 };
 

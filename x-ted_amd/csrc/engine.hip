@@ -81,7 +81,7 @@ Context::~Context() {
   for (Group& g : batch_) g.release();
   for (DevBuf* b : {&parent_, &label_, &orl_, &kdep_, &kdn_, &lrank_, &kr_, &krkd_, &krsize_,
                     &krleaf_, &trees_, &prep_scratch_, &prep_off_, &skr_, &skd_, &ssize_, &sleaf_,
-                    &spre_, &kdoff_, &rtop_})
+                    &spre_, &kdoff_, &rtop_, &cdel_, &cins_, &cren_, &pdel_, &pins_, &ctot_})
     b->release();
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
@@ -145,6 +145,8 @@ PoolView Context::view() const {
   v.spre = spre_.as<int32_t>();
   v.kdoff = kdoff_.as<int32_t>();
   v.rtop = rtop_.as<int32_t>();
+  v.pdel = pdel_.as<int32_t>();
+  v.pins = pins_.as<int32_t>();
   v.trees = trees_.as<TreeMeta>();
   return v;
 }
@@ -152,7 +154,8 @@ PoolView Context::view() const {
 int Context::grow_pool(int64_t nodes) {
   const size_t bytes = (size_t)nodes * sizeof(int32_t);
   for (DevBuf* b : {&parent_, &label_, &orl_, &kdep_, &kdn_, &lrank_, &kr_, &krkd_, &krsize_,
-                    &krleaf_, &skr_, &skd_, &ssize_, &sleaf_, &spre_, &kdoff_, &rtop_})
+                    &krleaf_, &skr_, &skd_, &ssize_, &sleaf_, &spre_, &kdoff_, &rtop_, &pdel_,
+                    &pins_})
     ST_TRY(b->grow(bytes, true, stream_));
   return OK;
 }
@@ -213,6 +216,13 @@ int Context::pool_add(int32_t ntrees, const int32_t* sizes, const int32_t* paren
     compute_orl(parents + in_off, n, orl.data());
     ti[t].n = n;
     ti[t].ksum = keyroot_size_sum(orl.data(), keyroots_desc(parents + in_off, orl.data(), n));
+    ti[t].maxlab = *std::max_element(labels + in_off, labels + in_off + n);
+    ti[t].minlab = *std::min_element(labels + in_off, labels + in_off + n);
+    if (cost_alphabet_ > 0 && (ti[t].minlab < 0 || ti[t].maxlab >= cost_alphabet_)) {
+      set_error("pool_add: tree " + std::to_string(t) + " has a label outside the cost alphabet [0, " +
+                std::to_string(cost_alphabet_) + ")");
+      return ERR_INVALID_ARG;
+    }
     in_off += n;
   }
   const int32_t first = (int32_t)meta_.size();
@@ -261,7 +271,78 @@ int Context::pool_add(int32_t ntrees, const int32_t* sizes, const int32_t* paren
     if (ids) ids[t] = first + t;
   }
   pool_nodes_ += total;
+  if (cost_alphabet_ > 0) return cost_prefix(first, ntrees);
   return OK;
+}
+
+int Context::cost_prefix(int32_t first, int32_t ntrees) {
+  if (ntrees <= 0) return OK;
+  ST_TRY(ctot_.grow(sizeof(int32_t) * 2 * ntrees, false, stream_));
+  CostPrefixArgs ca;
+  ca.trees = trees_.as<TreeMeta>();
+  ca.label = label_.as<int32_t>();
+  ca.del = cdel_.as<int32_t>();
+  ca.ins = cins_.as<int32_t>();
+  ca.pdel = pdel_.as<int32_t>();
+  ca.pins = pins_.as<int32_t>();
+  ca.totals = ctot_.as<int32_t>();
+  ca.first_tree = first;
+  ca.alphabet = cost_alphabet_;
+  launch_cost_prefix(ca, ntrees, stream_);
+  HIP_TRY(hipGetLastError());
+  std::vector<int32_t> tot(2 * ntrees);
+  HIP_TRY(hipMemcpyAsync(tot.data(), ctot_.p, sizeof(int32_t) * 2 * ntrees, hipMemcpyDeviceToHost,
+                         stream_));
+  HIP_TRY(hipStreamSynchronize(stream_));
+  for (int32_t t = 0; t < ntrees; ++t) {
+    info_[first + t].sdel = tot[2 * t];
+    info_[first + t].sins = tot[2 * t + 1];
+  }
+  return OK;
+}
+
+int Context::set_costs(int32_t alphabet, const int32_t* del, const int32_t* ins,
+                       const int32_t* ren) {
+  std::lock_guard<std::mutex> lk(mu_);
+  if (!del) {
+    cost_alphabet_ = 0;
+    return OK;
+  }
+  if (alphabet < 1 || alphabet > 4096 || !ins || !ren) {
+    set_error("set_costs: alphabet must be 1 to 4096 with del, ins and ren given");
+    return ERR_INVALID_ARG;
+  }
+  const int64_t a2 = (int64_t)alphabet * alphabet;
+  auto bad = [](const int32_t* p, int64_t k) {
+    for (int64_t i = 0; i < k; ++i)
+      if (p[i] < 0 || p[i] > 32767) return true;
+    return false;
+  };
+  if (bad(del, alphabet) || bad(ins, alphabet) || bad(ren, a2)) {
+    set_error("set_costs: costs must lie in [0, 32767]");
+    return ERR_INVALID_ARG;
+  }
+  HIP_TRY(hipSetDevice(device_));
+  ST_TRY(cdel_.grow(sizeof(int32_t) * alphabet, false, stream_));
+  ST_TRY(cins_.grow(sizeof(int32_t) * alphabet, false, stream_));
+  ST_TRY(cren_.grow(sizeof(int32_t) * a2, false, stream_));
+  HIP_TRY(hipMemcpyAsync(cdel_.p, del, sizeof(int32_t) * alphabet, hipMemcpyHostToDevice, stream_));
+  HIP_TRY(hipMemcpyAsync(cins_.p, ins, sizeof(int32_t) * alphabet, hipMemcpyHostToDevice, stream_));
+  HIP_TRY(hipMemcpyAsync(cren_.p, ren, sizeof(int32_t) * a2, hipMemcpyHostToDevice, stream_));
+  HIP_TRY(hipStreamSynchronize(stream_));
+  hdel_.assign(del, del + alphabet);
+  hins_.assign(ins, ins + alphabet);
+  cost_alphabet_ = alphabet;
+  return cost_prefix(0, (int32_t)meta_.size());
+}
+
+int64_t Context::whole_tree_cost(const int32_t* labels, int32_t n, bool del) const {
+  int64_t s = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    if (labels[i] < 0 || labels[i] >= cost_alphabet_) return -1;
+    s += del ? hdel_[labels[i]] : hins_[labels[i]];
+  }
+  return s;
 }
 
 int Context::tree_arrays(int32_t id, int32_t* orl, int32_t* kr, int32_t* K, int32_t* kd) {
@@ -347,6 +428,9 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   g.leaf_tables = g.leaf_cells = 0;
   int32_t nlevels = 0;
   g.maxn = 1;
+  // The leaf and tiny forms assume unit costs: with a cost model every table
+  // takes the row-scan forms.
+  const int32_t le = cost_alphabet_ > 0 ? 0 : lane_enable;
   for (int32_t i = 0; i < npairs; ++i) {
     const int32_t x = xs[i], y = ys[i];
     if (x < 0 || y < 0 || x >= (int32_t)meta_.size() || y >= (int32_t)meta_.size()) {
@@ -355,15 +439,29 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
     }
     const TreeMeta &a = meta_[x], &b = meta_[y];
     const int64_t mn = (int64_t)a.n * b.n;
+    // Far values sit in 16-bit LDS slots: every forest value (at most
+    // deleting all of x and inserting all of y) must stay below 2^16.
+    if (cost_alphabet_ > 0 && (info_[x].minlab < 0 || info_[x].maxlab >= cost_alphabet_ ||
+                               info_[y].minlab < 0 || info_[y].maxlab >= cost_alphabet_)) {
+      set_error("run_batch: pair " + std::to_string(i) +
+                " uses a pool tree with labels outside the cost alphabet");
+      return ERR_INVALID_ARG;
+    }
+    if (cost_alphabet_ > 0 && info_[x].sdel + info_[y].sins > 65535) {
+      set_error("run_batch: pair " + std::to_string(i) + " can cost " +
+                std::to_string(info_[x].sdel + info_[y].sins) +
+                " under the cost model; forest values above 65535 are not supported");
+      return ERR_TOO_LARGE;
+    }
     g.maxn = std::max(g.maxn, std::max(a.n, b.n));
     g.pm[i] = PairMeta{x, y, td_total + kTdPad, td_total + 2 * kTdPad + mn};
     td_total += 2 * mn + 3 * kTdPad;
     // With the leaf kernel on, the plan covers only non-leaf keyroot pairs;
     // the leaf kernel writes one line per leaf keyroot of either tree, in
     // tiles of up to 64 lines of one side.
-    const int32_t ka = lane_enable ? a.K - a.K1 : a.K, kb = lane_enable ? b.K - b.K1 : b.K;
-    g.lines[i + 1] = g.lines[i] + (lane_enable ? (a.n + 63) / 64 + (b.n + 63) / 64 : 0);
-    if (lane_enable) {
+    const int32_t ka = le ? a.K - a.K1 : a.K, kb = le ? b.K - b.K1 : b.K;
+    g.lines[i + 1] = g.lines[i] + (le ? (a.n + 63) / 64 + (b.n + 63) / 64 : 0);
+    if (le) {
       g.leaf_tables += (int64_t)a.K1 * b.K + (int64_t)(a.K - a.K1) * b.K1;
       g.leaf_cells += (int64_t)a.K1 * info_[y].ksum + (int64_t)b.K1 * (info_[x].ksum - a.K1);
     }
@@ -416,9 +514,10 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   pa.lchunk_tables = plan_chunk;
   pa.rule = rule;
   pa.rule_tail = ClassRule{lds_limit, 1, 1 << 30, lds_limit, 16, rule.xl_seq_rows};
+  if (cost_alphabet_ > 0) pa.rule.tiny = pa.rule_tail.tiny = 0;
   pa.tail_level = g.tail_lv.as<uint8_t>();
-  pa.force_class = force_class;
-  pa.lane_enable = lane_enable;
+  pa.force_class = cost_alphabet_ > 0 && force_class == kClassTiny ? -1 : force_class;
+  pa.lane_enable = le;
   pa.counts = g.counts.as<unsigned long long>();
   pa.cells = g.cells.as<unsigned long long>();
   pa.cursor = g.cursor.as<unsigned long long>();
@@ -568,6 +667,9 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
       da.xl_epoch = 0;
       da.xl_nbmax = 1;
       da.xl_seq_rows = rule.xl_seq_rows;
+      da.ren = cren_.as<int32_t>();
+      da.alphabet = cost_alphabet_;
+      da.costs = cost_alphabet_ > 0 ? 1 : 0;
       if (w.cls >= kFirstXLClass) {
         const int bw = xl_bw(w.cls - kFirstXLClass);
         da.xl_off = g.xl_off.as<int64_t>() + g.xl_first[w.bucket];
@@ -664,6 +766,8 @@ std::vector<uint64_t> Context::graph_signature(const Group& g, int lane) const {
   s.insert(s.end(), pv, pv + sizeof(PoolView) / sizeof(uint64_t));
   s.insert(s.end(), g.h_counts.begin(), g.h_counts.end());
   for (int32_t m : g.h_maxlds) s.push_back((uint64_t)(uint32_t)m);
+  s.push_back((uint64_t)cost_alphabet_);
+  s.push_back((uint64_t)(uintptr_t)cren_.p);
   return s;
 }
 

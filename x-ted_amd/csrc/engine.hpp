@@ -96,7 +96,9 @@ struct ProfLaunch {
 
 struct HostTreeInfo {
   int32_t n = 0;
-  int64_t ksum = 0;  // S = sum of keyroot subtree sizes
+  int64_t ksum = 0;    // S = sum of keyroot subtree sizes
+  int32_t minlab = 0, maxlab = 0;  // smallest and largest label id
+  int64_t sdel = 0, sins = 0;  // cost mode: deleting / inserting the whole tree
 };
 
 class Context {
@@ -111,6 +113,15 @@ class Context {
   int pool_add(int32_t ntrees, const int32_t* sizes, const int32_t* parents,
                const int32_t* labels, int32_t* ids);
   int pool_clear();
+  // Non-unit costs per label (labels in [0, alphabet), each cost in
+  // [0, 32767]): del, ins (alphabet entries) and ren (alphabet x alphabet,
+  // row = x label). del == nullptr restores unit costs.
+  int set_costs(int32_t alphabet, const int32_t* del, const int32_t* ins, const int32_t* ren);
+  bool cost_mode() const { return cost_alphabet_ > 0; }
+  // Cost of deleting (del) or inserting every node of a tree under the cost
+  // model: the distance between it and the empty tree. -1 for a label outside
+  // the alphabet.
+  int64_t whole_tree_cost(const int32_t* labels, int32_t n, bool del) const;
   int32_t pool_size() const { return (int32_t)meta_.size(); }
   const TreeMeta& tree_meta(int32_t id) const { return meta_[id]; }
   const HostTreeInfo& tree_info(int32_t id) const { return info_[id]; }
@@ -195,6 +206,10 @@ class Context {
   uint32_t xl_epoch_ = 0;  // tag of the latest column-blocked launch
   std::vector<hipEvent_t> prof_ev_;
   int grow_pool(int64_t nodes);
+  int cost_prefix(int32_t first, int32_t ntrees);  // pdel_ / pins_ and totals of those trees
+  int32_t cost_alphabet_ = 0;                       // 0: unit costs
+  std::vector<int32_t> hdel_, hins_;                // host copies of the per-label costs
+  DevBuf cdel_, cins_, cren_, pdel_, pins_, ctot_;
   std::vector<uint64_t> graph_signature(const Group& g, int lane) const;
   int run_levels(Group& g, int lane, std::vector<ProfLaunch>* prof);
   PoolView view() const;

@@ -326,7 +326,7 @@ __device__ __forceinline__ int64_t xl_words(const XLGeom& g) {
 // reads with one coalesced E-int load per lane, and the row scan is G
 // chained wave scans over groups.
 // ---------------------------------------------------------------------------
-template <int W, int C, bool kXL>
+template <int W, int C, bool kXL, bool kCost>
 __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   constexpr int E = C <= 8 ? C : 8;  // contiguous columns per group
   constexpr int G = C / E;           // groups per lane
@@ -387,6 +387,22 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   const int32_t nrows = rmax - r0;
   const int32_t base = kdepR[r0];
   const int32_t orl_c0 = cmax - 1;
+  // Non-unit costs (kCost). Hpre / Vpre: prefix sums of the horizontal
+  // (column) and vertical (row) edit costs over the column and row trees:
+  // insertions and deletions when the rows are x, the other way round when
+  // they are y. Q(c) = Hpre[cmax-1] - Hpre[c] takes the place of t (with
+  // unit costs Q(c) = t), hq = Hpre[cmax-1], hc = Hpre[cmax] (so hc - hq is
+  // the cost of column cmax-1) and vr = Vpre[rmax].
+  const int32_t* __restrict__ Hpre = nullptr;
+  const int32_t* __restrict__ Vpre = nullptr;
+  int32_t hq = 0, hc = 0, vr = 0;
+  if constexpr (kCost) {
+    Hpre = (rows_x ? A.pool.pins : A.pool.pdel) + offC;
+    Vpre = (rows_x ? A.pool.pdel : A.pool.pins) + offR;
+    hq = Hpre[cmax - 1];
+    hc = Hpre[cmax];
+    vr = Vpre[rmax];
+  }
   int32_t nb = 1;
   bool seq = false;  // one workgroup sweeps every block of the table
   if constexpr (kXL) {
@@ -443,6 +459,8 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // their stores are masked.
   int32_t bfar[C], labc[C], foff[C], woff[C], Fpm[C], negt[C], tdq[S][C];
   int32_t xinf[kXL ? C : 1];
+  // F[rmax][c] + (vertical cost of row rmax-1) - Q(c): 2 with unit costs.
+  const int32_t fpm0 = kCost ? (hc - hq) + (vr - Vpre[rmax - 1]) : 2;
 #pragma unroll
   for (int g = 0; g < G; ++g) {
 #pragma unroll
@@ -456,13 +474,13 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       const int32_t ocm = orlC[max(c - 1, c0)];
       const int32_t lrm = lrankC[max(c - 1, c0)];
       const bool tree = (oc == orl_c0);
-      bfar[j] = tree ? -1 : cmax - oc - 1;
+      bfar[j] = tree ? -1 : kCost ? hc - Hpre[oc + 1] : cmax - oc - 1;
       labc[j] = lab;
       foff[j] = tree ? wb : lro - lbase;
       const bool writes = t < ncols && c > c0 && ocm == c - 1;
       woff[j] = writes ? lrm - lbase : -1;
-      Fpm[j] = 2;  // F[rmax][c] + 1 - t = cmax - c + 1 - t
-      negt[j] = -t;
+      Fpm[j] = fpm0;  // F[rmax][c] + 1 - t = cmax - c + 1 - t with unit costs
+      negt[j] = kCost ? Hpre[c] - hq : -t;
       // Opaque to the optimiser, so that sums with -t stay three-operand
       // adds (v_add3_u32) instead of becoming an add and a subtract.
       asm("" : "+v"(negt[j]));
@@ -532,7 +550,8 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // (label, Rf = orl_R(r)+1, slots = fslot | wslot << 16 | wrow << 31) and,
   // for kXL, the saved row's index (leaf rank of r-1) or -1; read per row
   // with readlane; the next chunk is loaded one chunk ahead.
-  auto load_meta = [&](int q, int32_t& mlab, int32_t& mrf, int32_t& mslot, int32_t& mrid) {
+  auto load_meta = [&](int q, int32_t& mlab, int32_t& mrf, int32_t& mslot, int32_t& mrid,
+                       int32_t& mvp) {
     const int32_t rr = rmax - 1 - (64 * q + lane);
     const int32_t r = max(rr, r0);  // rows past the table read row r0
     const int32_t rp = max(r - 1, r0);
@@ -545,11 +564,13 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     mrf = orr + 1;
     mslot = fs | ws;
     if constexpr (kXL) mrid = (saved && rr >= r0) ? lrankR[rp] - lrankR[r0] : -1;
+    if constexpr (kCost) mvp = Vpre[r];
   };
   const int32_t nchunks = (nrows + 63) >> 6;
   int32_t mlab, mrf, mslot, mrid = -1, nlab = 0, nrf = 0, nslot = 0, nrid = -1;
-  load_meta(0, mlab, mrf, mslot, mrid);
-  if (nchunks > 1) load_meta(1, nlab, nrf, nslot, nrid);
+  int32_t mvp = 0, nvp = 0;  // kCost: Vpre of the row
+  load_meta(0, mlab, mrf, mslot, mrid, mvp);
+  if (nchunks > 1) load_meta(1, nlab, nrf, nslot, nrid, nvp);
 
   // Column-blocked form: the upstream carry (crv) and this block's imports
   // (irv, wave 0 only) of row k are read into ring slot k % S when row k-PD
@@ -624,7 +645,11 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // F[r+1] at the column just before this wave's block (t = tb0 + w*64*C - 1)
   // for the row about to start: the boundary row's t + 1 first, then each
   // row's carry-in plus that t.
-  int32_t left_prev = tb0 + wave * 64 * C;
+  // (With costs: Q at that column, qleft, and F[rmax] there.)
+  const int32_t cleft = max(cmax - tb0 - wave * 64 * C, c0);
+  const int32_t qleft = kCost ? hq - Hpre[cleft] : tb0 + wave * 64 * C - 1;
+  int32_t left_prev = kCost ? hc - Hpre[cleft] : tb0 + wave * 64 * C;
+  int32_t vp_prev = vr;  // kCost: Vpre of the previous row
 
   // Rows go S at a time in one flat loop with the inner loop fully unrolled,
   // so row k of the table always uses ring slot k % S and every slot index is
@@ -638,7 +663,8 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       mrf = nrf;
       mslot = nslot;
       mrid = nrid;
-      if ((k0 >> 6) + 1 < nchunks) load_meta((k0 >> 6) + 1, nlab, nrf, nslot, nrid);
+      mvp = nvp;
+      if ((k0 >> 6) + 1 < nchunks) load_meta((k0 >> 6) + 1, nlab, nrf, nslot, nrid, nvp);
     }
 #pragma unroll
     for (int u = 0; u < S; ++u) {
@@ -661,6 +687,18 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       const int32_t Rf = __builtin_amdgcn_readlane(mrf, i);
       const int32_t sl = __builtin_amdgcn_readlane(mslot, i);
       const bool tree_row = (Rf == rmax);
+      // Vertical costs of this row and of the next, and F[r][cmax] (unit
+      // costs: 1, 1 and rmax - r).
+      int32_t vcur = 1, vnext = 1, frc = rmax - r;
+      if constexpr (kCost) {
+        const int32_t vpr = __builtin_amdgcn_readlane(mvp, i);
+        const int32_t vpn = i < 63 ? __builtin_amdgcn_readlane(mvp, i + 1)
+                                   : __builtin_amdgcn_readlane(nvp, 0);
+        vcur = vp_prev - vpr;
+        vnext = vpr - vpn;
+        frc = vr - vpr;
+        vp_prev = vpr;
+      }
       const uint16_t* frow = slots + (sl & 0xFFFF) * rowlen;
       const bool wrow = sl < 0;
       uint16_t* wrow_ptr = slots + ((sl >> 16) & 0x7FFF) * rowlen;
@@ -685,8 +723,8 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           for (int g = 0; g < G; ++g) {
             const int jl = g * E + E - 1;  // the group's last column
             const int32_t fill =
-                g ? __builtin_amdgcn_readlane(Fpm[jl - E] - 1 - negt[jl - E], 63) : left_prev;
-            fpr0[g] = wave_shr1(Fpm[jl] - 1 - negt[jl], fill);
+                g ? __builtin_amdgcn_readlane(Fpm[jl - E] - vcur - negt[jl - E], 63) : left_prev;
+            fpr0[g] = wave_shr1(Fpm[jl] - vcur - negt[jl], fill);
           }
         }
 #pragma unroll
@@ -699,8 +737,15 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
               // diagonal F[r+1][c+1] + rename on a tree column, else the
               // boundary-row far value. F[r+1][c+1] - t = Fpm[j-1] - 2.
               // Blended with a mask rather than a branch per element.
-              const int32_t fprm = e ? Fpm[j - 1] - 2 : fpr0[g] + negt[j];
-              const int32_t diag = fprm + (int32_t)(labr != labc[j]);
+              int32_t fprm, rename;
+              if constexpr (kCost) {
+                fprm = e ? Fpm[j - 1] - vcur - negt[j - 1] + negt[j] : fpr0[g] + negt[j];
+                rename = A.ren[rows_x ? labr * A.alphabet + labc[j] : labc[j] * A.alphabet + labr];
+              } else {
+                fprm = e ? Fpm[j - 1] - 2 : fpr0[g] + negt[j];
+                rename = (int32_t)(labr != labc[j]);
+              }
+              const int32_t diag = fprm + rename;
               const int32_t far = bfar[j] + tdq[u][j] + negt[j];
               const int32_t m = bfar[j] >> 31;  // all ones on a tree column
               dm = (diag & m) | (far & ~m);
@@ -723,7 +768,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       // Carry into this wave: the boundary F[r][cmax] - (-1), or for a
       // later column block the carry of the blocks before it, then the
       // totals of this workgroup's earlier waves.
-      int32_t carry = (rmax - r) + 1;
+      int32_t carry = kCost ? frc + (hc - hq) : (rmax - r) + 1;
       if constexpr (kXL) {
         if (blk > 0) {
           carry = (crok >> u) & 1u ? crv[u]
@@ -748,7 +793,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       // wave's last element ends with h = min(its carry-in, its own total) =
       // this wave's carry-in, and F = h + t; for wave 0 of block 0 this gives
       // F[r][cmax] = rmax - r.
-      left_prev = carry + (tb0 + wave * 64 * C - 1);
+      left_prev = carry + qleft;
       // h becomes G - t; Fpm = G + 1 - t = h + 1.
 #pragma unroll
       for (int g = 0; g < G; ++g) {
@@ -757,7 +802,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
         for (int e = 0; e < E; ++e) {
           const int j = g * E + e;
           h[j] = min(cg, h[j]);
-          Fpm[j] = h[j] + 1;
+          Fpm[j] = h[j] + vnext;
         }
       }
 
@@ -768,8 +813,8 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           const int32_t t = TCOL(j / E, j % E);
           if (t < ncols && bfar[j] < 0) {
             const int32_t c = cmax - 1 - t;
-            TDo[(int64_t)r * ldo + c] = h[j] + t;
-            TDt[(int64_t)c * ldt + r] = h[j] + t;
+            TDo[(int64_t)r * ldo + c] = h[j] - negt[j];
+            TDt[(int64_t)c * ldt + r] = h[j] - negt[j];
           }
         }
       }
@@ -781,7 +826,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           uint16_t* dst = woff[j] >= 0 ? wrow_ptr + woff[j] : sink + lane;
           *dst = (uint16_t)(h[j] - negt[j]);
         }
-        if (threadIdx.x == 0) wrow_ptr[wb] = (uint16_t)(rmax - r);
+        if (threadIdx.x == 0) wrow_ptr[wb] = (uint16_t)frc;
         if constexpr (kXL) {
           // Imported far values of this saved row; far values later
           // blocks import from this one.
@@ -1024,8 +1069,12 @@ void launch_xl_zero(unsigned long long* p, int64_t n, hipStream_t s) {
 // ---------------------------------------------------------------------------
 template <int W, int C>
 static void launch_row(const DPArgs& a, int lds_bytes, hipStream_t s) {
-  hipLaunchKernelGGL((rowscan_kernel<W, C, false>), dim3((unsigned)a.ntasks), dim3(64 * W),
-                     (size_t)lds_bytes, s, a);
+  if (a.costs)
+    hipLaunchKernelGGL((rowscan_kernel<W, C, false, true>), dim3((unsigned)a.ntasks), dim3(64 * W),
+                       (size_t)lds_bytes, s, a);
+  else
+    hipLaunchKernelGGL((rowscan_kernel<W, C, false, false>), dim3((unsigned)a.ntasks),
+                       dim3(64 * W), (size_t)lds_bytes, s, a);
 }
 
 template <int W>
@@ -1041,15 +1090,23 @@ static void launch_row_c(int ci, const DPArgs& a, int lds, hipStream_t s) {
 
 template <int W, int C>
 static void launch_xl(const DPArgs& a, int lds_bytes, hipStream_t s) {
-  hipLaunchKernelGGL((rowscan_kernel<W, C, true>), dim3((unsigned)(a.ntasks * a.xl_nbmax)),
-                     dim3(64 * W), (size_t)lds_bytes, s, a);
+  const dim3 grid((unsigned)(a.ntasks * a.xl_nbmax));
+  if (a.costs)
+    hipLaunchKernelGGL((rowscan_kernel<W, C, true, true>), grid, dim3(64 * W), (size_t)lds_bytes,
+                       s, a);
+  else
+    hipLaunchKernelGGL((rowscan_kernel<W, C, true, false>), grid, dim3(64 * W), (size_t)lds_bytes,
+                       s, a);
 }
 
 // Grants every row-scan instantiation the LDS it may ask for (> 64 KiB needs
 // an explicit attribute).
 template <int W, int C, bool kXL = false>
 static hipError_t allow_lds(int limit) {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&rowscan_kernel<W, C, kXL>),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&rowscan_kernel<W, C, kXL, false>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, limit);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&rowscan_kernel<W, C, kXL, true>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, limit);
 }
 template <int W>

@@ -32,6 +32,22 @@ struct PrepArgs {
 };
 void launch_tree_prep(const PrepArgs& a, int ntrees, hipStream_t s);
 
+// Cost prefix sums of trees [first_tree, first_tree + ntrees): per node i in
+// [0, n], pdel[off + i] / pins[off + i] = the deletion / insertion costs of
+// nodes [0, i) by label; totals[2t], totals[2t+1] = the whole tree's.
+struct CostPrefixArgs {
+  const TreeMeta* trees;
+  const int32_t* label;
+  const int32_t* del;  // per label
+  const int32_t* ins;
+  int32_t* pdel;
+  int32_t* pins;
+  int32_t* totals;     // 2 per tree of this launch
+  int32_t first_tree;
+  int32_t alphabet;    // labels outside [0, alphabet) read the last entry
+};
+void launch_cost_prefix(const CostPrefixArgs& a, int ntrees, hipStream_t s);
+
 // Plan builder: count, scan and scatter of (level, class) buckets.
 struct PlanArgs {
   PoolView pool;
@@ -107,6 +123,11 @@ struct DPArgs {
   uint32_t xl_epoch;             // tag of this launch's words (never 0)
   int32_t xl_nbmax;              // workgroups per task in the grid
   int32_t xl_seq_rows;           // tables this short: one workgroup sweeps all blocks
+  // Non-unit costs (costs != 0): the pool's pdel / pins prefix sums carry
+  // the deletion and insertion costs; ren[a * alphabet + b] relabels a to b.
+  const int32_t* ren;
+  int32_t alphabet;
+  int32_t costs;
 };
 // Arena layout of the column-blocked tasks of a batch (after the scatter).
 struct XLLayoutArgs {

@@ -256,6 +256,49 @@ __global__ __launch_bounds__(kPrepThreads) void tree_prep_kernel(PrepArgs A) {
   }
 }
 
+__global__ __launch_bounds__(kPrepThreads) void cost_prefix_kernel(CostPrefixArgs A) {
+  __shared__ int32_t lds_tmp[kPrepThreads];
+  const TreeMeta meta = A.trees[A.first_tree + blockIdx.x];
+  const int32_t n = meta.n;
+  const int32_t* lab = A.label + meta.node_off;
+  // A pool tree added before the cost model may hold labels outside its
+  // alphabet; such a tree's sums are meaningless (pairs using it are
+  // rejected on the host) but its reads stay inside the cost arrays.
+  const uint32_t amax = (uint32_t)A.alphabet - 1;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int32_t* cost = pass ? A.ins : A.del;
+    int32_t* out = (pass ? A.pins : A.pdel) + meta.node_off;
+    // Each thread owns a contiguous chunk of nodes; a scan of the chunk sums.
+    const int per = (n + kPrepThreads - 1) / kPrepThreads;
+    const int lo = min(n, (int)threadIdx.x * per), hi = min(n, lo + per);
+    int32_t s = 0;
+    for (int i = lo; i < hi; ++i) s += cost[min((uint32_t)lab[i], amax)];
+    lds_tmp[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < kPrepThreads; off <<= 1) {
+      const int32_t v = (int)threadIdx.x >= off ? lds_tmp[threadIdx.x - off] : 0;
+      __syncthreads();
+      lds_tmp[threadIdx.x] += v;
+      __syncthreads();
+    }
+    int32_t run = threadIdx.x ? lds_tmp[threadIdx.x - 1] : 0;
+    for (int i = lo; i < hi; ++i) {
+      out[i] = run;
+      run += cost[min((uint32_t)lab[i], amax)];
+    }
+    if (threadIdx.x == kPrepThreads - 1) {
+      out[n] = lds_tmp[kPrepThreads - 1];
+      A.totals[2 * blockIdx.x + pass] = lds_tmp[kPrepThreads - 1];
+    }
+    __syncthreads();
+  }
+}
+
+void launch_cost_prefix(const CostPrefixArgs& a, int ntrees, hipStream_t s) {
+  if (ntrees <= 0) return;
+  hipLaunchKernelGGL(cost_prefix_kernel, dim3(ntrees), dim3(kPrepThreads), 0, s, a);
+}
+
 void launch_tree_prep(const PrepArgs& a, int ntrees, hipStream_t s) {
   if (ntrees <= 0) return;
   hipLaunchKernelGGL(tree_prep_kernel, dim3(ntrees), dim3(kPrepThreads), 0, s, a);

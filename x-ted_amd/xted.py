@@ -78,6 +78,7 @@ _SIGNATURES = {
     "xted_set_level_streams": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_batch_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "xted_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "xted_set_costs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32P, _I32P, _I32P]),
     "xted_set_tail_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_xl_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_class_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
@@ -320,6 +321,18 @@ class Context:
 
     def set_graphs(self, on=True):
         _check(lib().xted_set_graphs(self._h, 1 if on else 0))
+
+    def set_costs(self, costs=None):
+        """costs = (dele, ins, ren) per-label int32 arrays (alphabet, alphabet,
+        alphabet x alphabet with rows = x labels), or None for unit costs."""
+        if costs is None:
+            _check(lib().xted_set_costs(self._h, 0, None, None, None))
+            return
+        dele, ins, ren = (np.ascontiguousarray(a, dtype=np.int32) for a in costs)
+        a = len(dele)
+        if len(ins) != a or ren.shape != (a, a):
+            raise ValueError("costs: dele and ins of one length, ren square of it")
+        _check(lib().xted_set_costs(self._h, a, _p(dele), _p(ins), _p(ren)))
 
     def set_profile(self, on=True):
         _check(lib().xted_set_profile(self._h, 1 if on else 0))
