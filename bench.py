@@ -16,6 +16,8 @@ The same JSON line also reports:
     HBM, timed with events around each of its launches;
   * the reference's own X-TED_CPU (compiled from its sources into
     oracle/_ref/) on a bounded sample of the same pairs, on this host;
+  * the same batch under a non-unit cost model (per-label delete / insert
+    costs and a rename matrix): pairs/s and its dominant kernel's roofline;
   * the extra-large pair (two 20,000-node random recursive trees): GCUPS
     and algorithmic HBM rate;
   * serving: a pool of 10,000 trees (50..5000 nodes, log-uniform), Poisson
@@ -208,6 +210,24 @@ def traffic_from_profile(kernel):
     return None
 
 
+def dominant_roofline(prof, suffix=""):
+    """Roofline of the class with the most device time in a profiled step
+    (4 bytes of TD per interior cell over its event-timed launches) and the
+    per-class table. suffix names the kernel variant in the profiles."""
+    cls, (launches, tables, cells, ms) = max(prof.items(), key=lambda kv: kv[1][3])
+    name = xted.class_name(cls) + suffix
+    achieved = BYTES_PER_CELL * cells / (ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profile(name),
+                "kernel": name, "launches": launches,
+                "avg_launch_ms": round(ms / launches, 4),
+                "bytes_per_launch": int(BYTES_PER_CELL * cells / launches)}
+    classes = {xted.class_name(c): {"launches": v[0], "tables": v[1], "cells": v[2],
+                                    "ms": round(v[3], 3)} for c, v in sorted(prof.items())}
+    return roofline, classes
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +257,8 @@ def main():
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--xl-only", action="store_true",
                     help="only the 20,000-node pair (for profiling it alone)")
+    ap.add_argument("--no-costs", action="store_true",
+                    help="skip the batch under a non-unit cost model")
     ap.add_argument("--serve-only", action="store_true",
                     help="only the serving leg (for profiling it alone)")
     args = ap.parse_args()
@@ -280,14 +302,14 @@ def main():
         dist, _, st = ctx.batch(xs, ys)
         log(f"warmup done: {st.tables} tables, {st.cells} cells, {st.levels} levels")
 
-    def timed(steps, profile, tag):
+    def timed(steps, profile, tag, expect):
         ctx.set_profile(profile)
         times, st = [], None
         for k in range(steps):
             t = time.perf_counter()
             d, _, st = ctx.batch(xs, ys)
             times.append(time.perf_counter() - t)
-            if not np.array_equal(d, dist):
+            if not np.array_equal(d, expect):
                 raise SystemExit("batched distances changed between steps")
             log(f"{tag} {k}: {times[-1] * 1e3:.1f} ms (plan {st.plan_ms:.1f} ms, "
                 f"dp {st.dp_ms:.1f} ms)")
@@ -301,24 +323,37 @@ def main():
     # --no-graphs. Then a second timed region with every forest launch
     # bracketed by HIP events on its own stream (launched one by one): the
     # per-class device times and the dominant kernel's launch durations.
-    times, _, st = timed(args.steps, False, "step")
-    ptimes, prof, _ = timed(max(1, args.profile_steps), True, "profiled step")
+    times, _, st = timed(args.steps, False, "step", dist)
+    ptimes, prof, _ = timed(max(1, args.profile_steps), True, "profiled step", dist)
     ms_per_step = 1e3 * sum(times) / len(times)
     pairs_per_s = args.pairs / (ms_per_step / 1e3)
     gcups = st.cells / (ms_per_step / 1e3) / 1e9
+    roofline, classes = dominant_roofline(prof)
 
-    # Dominant kernel: the class with the most device time in the last step.
-    dom = max(prof.items(), key=lambda kv: kv[1][3])
-    cls, (launches, tables, cells, ms) = dom
-    achieved = BYTES_PER_CELL * cells / (ms / 1e3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profile(xted.class_name(cls)),
-                "kernel": xted.class_name(cls), "launches": launches,
-                "avg_launch_ms": round(ms / launches, 4),
-                "bytes_per_launch": int(BYTES_PER_CELL * cells / launches)}
-    classes = {xted.class_name(c): {"launches": v[0], "tables": v[1], "cells": v[2],
-                                    "ms": round(v[3], 3)} for c, v in sorted(prof.items())}
+    # The same batch under a non-unit cost model: the row-scan forms with
+    # per-label cost lookups and prefix sums (the closed-form leaf tables and
+    # the tiny form are unit-cost only, so every table takes a row scan).
+    cost_leg = None
+    if not args.no_costs:
+        rng = np.random.default_rng(5)
+        dele = rng.integers(1, 9, 200).astype(np.int32)
+        ins = rng.integers(1, 9, 200).astype(np.int32)
+        ren = rng.integers(0, 17, (200, 200)).astype(np.int32)
+        np.fill_diagonal(ren, 0)
+        ctx.set_costs((dele, ins, ren))
+        cdist, _, _ = ctx.batch(xs, ys)
+        ctimes, _, cst = timed(args.steps, False, "cost step", cdist)
+        _, cprof, _ = timed(1, True, "profiled cost step", cdist)
+        ctx.set_costs(None)
+        cms = 1e3 * sum(ctimes) / len(ctimes)
+        croof, cclasses = dominant_roofline(cprof, "_cost")
+        cost_leg = {"costs": "alphabet 200, delete and insert uniform in [1, 8], rename uniform "
+                             "in [0, 16] (0 on equal labels), seed 5",
+                    "pairs_per_s": round(args.pairs / (cms / 1e3), 1),
+                    "ms_per_step": round(cms, 3), "gcups": round(cst.cells / cms / 1e6, 2),
+                    "tables": cst.tables, "vs_unit_costs": round(ms_per_step / cms, 3),
+                    "roofline": croof, "classes": cclasses}
+        log(f"cost model: {cost_leg}")
 
     # Plain per-pair drop-in path on the first pairs of the batch.
     n_plain = min(args.plain_pairs, args.pairs)
@@ -376,6 +411,7 @@ def main():
         "roofline": roofline,
         "classes": classes,
         "cpu_baseline": cpu,
+        "cost_model": cost_leg,
         "xl_pair": xl,
         "serving": serving,
     }

@@ -229,3 +229,26 @@ def test_pool_trees_outside_the_alphabet(cost_ctx):
     dist, tds, _ = cost_ctx.batch(ids[2:3], ids[3:4], td_shapes=[(70, 70)])
     od, otd = oracle.ted_costs(narrow[0], narrow[1], costs, threads=4)
     assert dist[0] == od and np.array_equal(tds[0], otd)
+
+
+def test_full_batch_under_costs_sampled(cost_ctx):
+    """The 4096-pair uneven batch (100..1000 nodes) under a cost model: every
+    table takes a row scan, so the first level's launches hold tens of
+    millions of workgroups and go in pieces. A sample of pairs is checked
+    against the oracle."""
+    import sys
+    from conftest import ROOT
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    import bench
+    trees = bench.batch_trees(4096)
+    cost_ctx.pool_clear()
+    ids = cost_ctx.pool_add(trees)
+    rng = np.random.default_rng(5)
+    costs = rand_costs(rng, 200)
+    cost_ctx.set_costs(costs)
+    dist, _, st = cost_ctx.batch(ids[0::2], ids[1::2])
+    assert st.tables > 1 << 28
+    for k in rng.choice(4096, 6, replace=False):
+        od, _ = oracle.ted_costs(trees[2 * k], trees[2 * k + 1], costs, threads=16, want_td=False)
+        assert dist[k] == od

@@ -25,6 +25,7 @@
 // stored as uint16.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <type_traits>
 #include <utility>
@@ -82,13 +83,13 @@ __device__ __forceinline__ int wave_shr1(int v, int fill) {
 // side, written by its tiles.
 // ---------------------------------------------------------------------------
 constexpr int kLeafTile = 64;
-__global__ __launch_bounds__(256) void leaf_kernel(LeafArgs A, int64_t ntiles) {
+__global__ __launch_bounds__(256) void leaf_kernel(LeafArgs A, int64_t ntiles, int64_t base) {
   // Row kLeafTile of the tile takes the values of lines past the tile's
   // count, which are computed (so a wave's lines have no branches between
   // them and their scans interleave) but never stored.
   __shared__ int32_t tile[kLeafTile + 1][kLeafTile + 1];
   __shared__ int32_t lines[kLeafTile];
-  const int64_t tix = blockIdx.x;
+  const int64_t tix = base + blockIdx.x;
   if (tix >= ntiles) return;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -170,9 +171,15 @@ __global__ __launch_bounds__(256) void leaf_kernel(LeafArgs A, int64_t ntiles) {
   }
 }
 
+// A grid's threads (gridDim.x * blockDim.x) must stay below 2^32: launches
+// over more work items than kGridThreads / threads-per-group go in pieces.
+constexpr int64_t kGridThreads = int64_t{1} << 31;
+
 void launch_leaf(const LeafArgs& a, int64_t ntiles, hipStream_t s) {
-  if (ntiles <= 0) return;
-  hipLaunchKernelGGL(leaf_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, a, ntiles);
+  constexpr int64_t kChunk = kGridThreads / 256;
+  for (int64_t base = 0; base < ntiles; base += kChunk)
+    hipLaunchKernelGGL(leaf_kernel, dim3((unsigned)std::min(kChunk, ntiles - base)), dim3(256), 0,
+                       s, a, ntiles, base);
 }
 
 // ---------------------------------------------------------------------------
@@ -332,9 +339,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   constexpr int G = C / E;           // groups per lane
   // TD ring slots (a power of two dividing 64): up to 32 registers of ring,
   // so narrow lanes prefetch more rows ahead.
-  // (The column-blocked form also rings its imports: at most 8 slots.)
+  // (The column-blocked form also rings its imports, and the cost form's
+  // longer row body has to unroll S times: at most 8 slots for both.)
   constexpr int S0 = C >= 16 ? 2 : C >= 8 ? 4 : C >= 4 ? 8 : 16;
-  constexpr int S = kXL && S0 > 8 ? 8 : S0;
+  constexpr int S = (kXL || kCost) && S0 > 8 ? 8 : S0;
   constexpr int PD = S - 1;  // rows of TD prefetched
   constexpr int BW = 64 * W * C;                    // columns of one block
   constexpr int KQ = kXL ? kMaxImports / 64 : 1;    // import entries per lane
@@ -1004,8 +1012,14 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
 
 void launch_tiny(const DPArgs& a, int slot_bytes, hipStream_t s) {
   const int slot_ints = (slot_bytes + 3) / 4;  // per table, in 4-byte units
-  hipLaunchKernelGGL(tiny_kernel, dim3((unsigned)((a.ntasks + kTinyPerGroup - 1) / kTinyPerGroup)),
-                     dim3(256), (size_t)kTinyPerGroup * slot_ints * 4, s, a, slot_ints);
+  constexpr int64_t kChunk = kGridThreads / 256 * kTinyPerGroup;
+  for (int64_t off = 0; off < a.ntasks; off += kChunk) {
+    DPArgs c = a;
+    c.tasks = a.tasks + off;
+    c.ntasks = std::min(kChunk, a.ntasks - off);
+    hipLaunchKernelGGL(tiny_kernel, dim3((unsigned)((c.ntasks + kTinyPerGroup - 1) / kTinyPerGroup)),
+                       dim3(256), (size_t)kTinyPerGroup * slot_ints * 4, s, c, slot_ints);
+  }
 }
 
 // Arena offsets of every column-blocked task of a batch, in bucket order
@@ -1069,12 +1083,19 @@ void launch_xl_zero(unsigned long long* p, int64_t n, hipStream_t s) {
 // ---------------------------------------------------------------------------
 template <int W, int C>
 static void launch_row(const DPArgs& a, int lds_bytes, hipStream_t s) {
-  if (a.costs)
-    hipLaunchKernelGGL((rowscan_kernel<W, C, false, true>), dim3((unsigned)a.ntasks), dim3(64 * W),
-                       (size_t)lds_bytes, s, a);
-  else
-    hipLaunchKernelGGL((rowscan_kernel<W, C, false, false>), dim3((unsigned)a.ntasks),
-                       dim3(64 * W), (size_t)lds_bytes, s, a);
+  // One table per workgroup: a long task list goes in pieces.
+  constexpr int64_t kChunk = kGridThreads / (64 * W);
+  for (int64_t off = 0; off < a.ntasks; off += kChunk) {
+    DPArgs c = a;
+    c.tasks = a.tasks + off;
+    c.ntasks = std::min(kChunk, a.ntasks - off);
+    if (a.costs)
+      hipLaunchKernelGGL((rowscan_kernel<W, C, false, true>), dim3((unsigned)c.ntasks),
+                         dim3(64 * W), (size_t)lds_bytes, s, c);
+    else
+      hipLaunchKernelGGL((rowscan_kernel<W, C, false, false>), dim3((unsigned)c.ntasks),
+                         dim3(64 * W), (size_t)lds_bytes, s, c);
+  }
 }
 
 template <int W>
@@ -1090,13 +1111,21 @@ static void launch_row_c(int ci, const DPArgs& a, int lds, hipStream_t s) {
 
 template <int W, int C>
 static void launch_xl(const DPArgs& a, int lds_bytes, hipStream_t s) {
-  const dim3 grid((unsigned)(a.ntasks * a.xl_nbmax));
-  if (a.costs)
-    hipLaunchKernelGGL((rowscan_kernel<W, C, true, true>), grid, dim3(64 * W), (size_t)lds_bytes,
-                       s, a);
-  else
-    hipLaunchKernelGGL((rowscan_kernel<W, C, true, false>), grid, dim3(64 * W), (size_t)lds_bytes,
-                       s, a);
+  // Workgroups take (task, block) tickets from one counter that the pieces
+  // of a long launch share: a later piece continues where the earlier one
+  // stopped, and every block it waits on holds a lower ticket, taken by a
+  // workgroup of an earlier (finished) piece or a running one of its own.
+  constexpr int64_t kChunk = kGridThreads / (64 * W);
+  const int64_t total = a.ntasks * a.xl_nbmax;
+  for (int64_t off = 0; off < total; off += kChunk) {
+    const dim3 grid((unsigned)std::min(kChunk, total - off));
+    if (a.costs)
+      hipLaunchKernelGGL((rowscan_kernel<W, C, true, true>), grid, dim3(64 * W),
+                         (size_t)lds_bytes, s, a);
+    else
+      hipLaunchKernelGGL((rowscan_kernel<W, C, true, false>), grid, dim3(64 * W),
+                         (size_t)lds_bytes, s, a);
+  }
 }
 
 // Grants every row-scan instantiation the LDS it may ask for (> 64 KiB needs

@@ -23,14 +23,16 @@ from collections import defaultdict
 
 
 def short(name):
-    """'rowscan_w4_c4', 'leaf_kernel', ... from a demangled kernel name."""
+    """'rowscan_w4_c4', 'xl_bw1024', 'rowscan_w4_c4_cost' (the variant under a
+    cost model), 'leaf_kernel', ... from a demangled kernel name."""
     base = name.split("(")[0].replace("void ", "").replace("xted::", "")
     if base.startswith("rowscan_kernel<"):
         parts = [p.strip() for p in base[len("rowscan_kernel<"):-1].split(",")]
         w, c = int(parts[0]), int(parts[1])
+        cost = "_cost" if len(parts) > 3 and parts[3] == "true" else ""
         if len(parts) > 2 and parts[2] == "true":
-            return f"xl_bw{64 * w * c}"
-        return f"rowscan_w{w}_c{c}"
+            return f"xl_bw{64 * w * c}{cost}"
+        return f"rowscan_w{w}_c{c}{cost}"
     return base
 
 
@@ -44,7 +46,10 @@ def kernels(path, bench=None):
         with open(bench) as f:
             for line in f:
                 if line.startswith("{"):
-                    classes = json.loads(line).get("classes", {})
+                    j = json.loads(line)
+                    classes = dict(j.get("classes", {}))
+                    cm = j.get("cost_model") or {}
+                    classes.update({k + "_cost": v for k, v in cm.get("classes", {}).items()})
     rows = sorted(per.items(), key=lambda kv: -sum(kv[1]))
     out = [f"{'kernel':<26}{'dispatches':>11}{'avg_us':>11}{'total_ms':>11}"
            f"{'alg_MB/launch':>15}{'alg_GB/s':>10}"]
