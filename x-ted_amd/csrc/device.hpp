@@ -41,6 +41,23 @@ struct PairMeta {
   int64_t tdt_off;  // TDT offset (n*m ints)
 };
 
+// What a forest kernel needs to start one table, written by the plan next to
+// the table's task word so that the kernel starts from one 64-byte read
+// instead of a chain of dependent reads (task, pair, trees, subtree ends).
+// The table is oriented with its short side as rows.
+struct alignas(16) TaskDesc {
+  int64_t tdo;       // offset of TD in the row layout ([r][c])
+  int64_t tdt;       // and in the column layout ([c][r])
+  int64_t offR;      // node offset of the row tree in the pool
+  int64_t offC;      // and of the column tree
+  int32_t ldo, ldt;  // row lengths of the two layouts
+  int32_t r0, c0;    // the table's keyroots (row tree, column tree)
+  int32_t rmax, cmax;
+  int32_t base_rx;   // kdep of r0; bit 31 set when the rows are x's nodes
+  int32_t lbase;     // leaves of the column tree before c0
+};
+static_assert(sizeof(TaskDesc) == 64, "TaskDesc is one 64-byte read");
+
 // Pointers to the pool arrays (device addresses).
 struct PoolView {
   const int32_t* parent;

@@ -382,8 +382,8 @@ void Group::release() {
   graph_exec = nullptr;
   graph = nullptr;
   graph_sig.clear();
-  for (DevBuf* b : {&pairs, &blk, &counts, &cells, &cursor, &maxlds, &tickets, &abort, &tasks, &td,
-                    &dist, &lines_d, &tail_lv, &xl_off, &xl_total, &xl_words})
+  for (DevBuf* b : {&pairs, &blk, &counts, &cells, &cursor, &maxlds, &tickets, &abort, &tasks,
+                    &descs, &td, &dist, &lines_d, &tail_lv, &xl_off, &xl_total, &xl_words})
     b->release();
 }
 
@@ -491,6 +491,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   ST_TRY(g.tickets.grow(sizeof(unsigned int) * nb, false, ps));
   ST_TRY(g.abort.grow(sizeof(int32_t) * 8, false, ps));
   ST_TRY(g.tasks.grow(sizeof(uint64_t) * std::max<int64_t>(planned, 1), false, ps));
+  ST_TRY(g.descs.grow(sizeof(TaskDesc) * std::max<int64_t>(planned, 1), false, ps));
   ST_TRY(g.td.grow((size_t)g.td_bytes, false, ps));
   ST_TRY(g.dist.grow(sizeof(int32_t) * npairs, false, ps));
   ST_TRY(g.lines_d.grow(sizeof(int64_t) * (npairs + 1), false, ps));
@@ -523,6 +524,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   pa.cursor = g.cursor.as<unsigned long long>();
   pa.maxlds = g.maxlds.as<int32_t>();
   pa.tasks = g.tasks.as<uint64_t>();
+  pa.descs = g.descs.as<TaskDesc>();
   pa.tickets = g.tickets.as<unsigned int>();
   pa.abort_flag = g.abort.as<int32_t>();
   launch_plan(pa, blk[npairs], ps);
@@ -659,6 +661,7 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
       da.pairs = g.pairs.as<PairMeta>();
       da.td = g.td.as<int32_t>();
       da.tasks = g.tasks.as<uint64_t>() + w.off;
+      da.descs = g.descs.as<TaskDesc>() + w.off;
       da.ntasks = (int64_t)w.cnt;
       da.xl_words = g.xl_words.as<unsigned long long>();
       da.xl_abort = g.abort.as<int32_t>();
@@ -759,7 +762,7 @@ std::vector<uint64_t> Context::graph_signature(const Group& g, int lane) const {
   s.push_back((uint64_t)g.npairs);
   s.push_back((uint64_t)g.nlevels);
   s.push_back((uint64_t)g.lines[g.npairs]);
-  for (const void* p : {g.pairs.p, g.td.p, g.tasks.p, g.lines_d.p})
+  for (const void* p : {g.pairs.p, g.td.p, g.tasks.p, g.descs.p, g.lines_d.p})
     s.push_back((uint64_t)(uintptr_t)p);
   const PoolView v = view();
   const uint64_t* pv = reinterpret_cast<const uint64_t*>(&v);

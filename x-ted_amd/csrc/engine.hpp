@@ -65,8 +65,8 @@ struct Group {
   std::vector<unsigned long long> h_counts, h_cells;
   std::vector<int32_t> h_maxlds;
   std::vector<uint8_t> h_tail;
-  DevBuf pairs, blk, counts, cells, cursor, maxlds, tickets, tasks, td, dist, lines_d, tail_lv,
-      xl_off, xl_total;
+  DevBuf pairs, blk, counts, cells, cursor, maxlds, tickets, tasks, descs, td, dist, lines_d,
+      tail_lv, xl_off, xl_total;
   DevBuf abort, xl_words;
   // finish_async: distances then the abort flag land in pinned memory
   int32_t* h_dist_pin = nullptr;

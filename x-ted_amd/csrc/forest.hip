@@ -366,15 +366,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     blk = tk % A.xl_nbmax;
   }
   if (tix >= A.ntasks) return;
-  const uint64_t task = A.tasks[tix];
-  const PairMeta pm = A.pairs[task_pair(task)];
-  const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
-  const int32_t i0 = (int32_t)task_i0(task), j0 = (int32_t)task_j0(task);
-  const int32_t a = A.pool.orl[tx.node_off + i0] - i0 + 1;
-  const int32_t b = A.pool.orl[ty.node_off + j0] - j0 + 1;
-  const bool rows_x = a <= b;
-  const int64_t offR = rows_x ? tx.node_off : ty.node_off;
-  const int64_t offC = rows_x ? ty.node_off : tx.node_off;
+  const TaskDesc dsc = A.descs[tix];
+  const bool rows_x = dsc.base_rx < 0;
+  const int64_t offR = dsc.offR;
+  const int64_t offC = dsc.offC;
   const int32_t* __restrict__ orlR = A.pool.orl + offR;
   const int32_t* __restrict__ labR = A.pool.label + offR;
   const int32_t* __restrict__ kdepR = A.pool.kdep + offR;
@@ -384,16 +379,16 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   const int32_t* __restrict__ lrankC = A.pool.lrank + offC;
   const int32_t* __restrict__ kdepC = A.pool.kdep + offC;
   const int32_t* __restrict__ rtopC = A.pool.rtop + offC;
-  const int32_t r0 = rows_x ? i0 : j0, c0 = rows_x ? j0 : i0;
-  int32_t* TDo = A.td + (rows_x ? pm.td_off : pm.tdt_off);  // [r][c]
-  int32_t* TDt = A.td + (rows_x ? pm.tdt_off : pm.td_off);  // [c][r]
-  const int64_t ldo = rows_x ? ty.n : tx.n;
-  const int64_t ldt = rows_x ? tx.n : ty.n;
+  const int32_t r0 = dsc.r0, c0 = dsc.c0;
+  int32_t* TDo = A.td + dsc.tdo;  // [r][c]
+  int32_t* TDt = A.td + dsc.tdt;  // [c][r]
+  const int64_t ldo = dsc.ldo;
+  const int64_t ldt = dsc.ldt;
 
-  const int32_t rmax = orlR[r0] + 1, cmax = orlC[c0] + 1;
+  const int32_t rmax = dsc.rmax, cmax = dsc.cmax;
   const int32_t ncols = cmax - c0;
   const int32_t nrows = rmax - r0;
-  const int32_t base = kdepR[r0];
+  const int32_t base = dsc.base_rx & 0x7FFFFFFF;
   const int32_t orl_c0 = cmax - 1;
   // Non-unit costs (kCost). Hpre / Vpre: prefix sums of the horizontal
   // (column) and vertical (row) edit costs over the column and row trees:
@@ -440,7 +435,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     kds = A.pool.kdn[offC + c0] + 1;
     kdep_c0 = kdepC[c0];
   } else {
-    lbase = lrankC[c0];
+    lbase = dsc.lbase;
     wb = lrankC[cmax] - lbase - 1;
   }
   const int32_t rowlen = wb + 1 + kds;
@@ -905,30 +900,24 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
   const int seg = threadIdx.x / kTinySeg;      // table of this lane in the workgroup
   const int64_t tix = (int64_t)blockIdx.x * kTinyPerGroup + seg;
   const bool live = tix < A.ntasks;
-  const uint64_t task = A.tasks[live ? tix : 0];  // idle segments redo task 0, storing nothing
-  const PairMeta pm = A.pairs[task_pair(task)];
-  const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
-  const int32_t i0 = (int32_t)task_i0(task), j0 = (int32_t)task_j0(task);
-  const int32_t a = A.pool.orl[tx.node_off + i0] - i0 + 1;
-  const int32_t b = A.pool.orl[ty.node_off + j0] - j0 + 1;
-  const bool rows_x = a <= b;
-  const int64_t offR = rows_x ? tx.node_off : ty.node_off;
-  const int64_t offC = rows_x ? ty.node_off : tx.node_off;
+  const TaskDesc dsc = A.descs[live ? tix : 0];  // idle segments redo task 0, storing nothing
+  const int64_t offR = dsc.offR;
+  const int64_t offC = dsc.offC;
   const int32_t* __restrict__ orlR = A.pool.orl + offR;
   const int32_t* __restrict__ labR = A.pool.label + offR;
   const int32_t* __restrict__ kdepR = A.pool.kdep + offR;
   const int32_t* __restrict__ orlC = A.pool.orl + offC;
   const int32_t* __restrict__ labC = A.pool.label + offC;
   const int32_t* __restrict__ lrankC = A.pool.lrank + offC;
-  const int32_t r0 = rows_x ? i0 : j0, c0 = rows_x ? j0 : i0;
-  int32_t* TDo = A.td + (rows_x ? pm.td_off : pm.tdt_off);  // [r][c]
-  int32_t* TDt = A.td + (rows_x ? pm.tdt_off : pm.td_off);  // [c][r]
-  const int64_t ldo = rows_x ? ty.n : tx.n;
-  const int64_t ldt = rows_x ? tx.n : ty.n;
-  const int32_t rmax = orlR[r0] + 1, cmax = orlC[c0] + 1;
+  const int32_t r0 = dsc.r0, c0 = dsc.c0;
+  int32_t* TDo = A.td + dsc.tdo;  // [r][c]
+  int32_t* TDt = A.td + dsc.tdt;  // [c][r]
+  const int64_t ldo = dsc.ldo;
+  const int64_t ldt = dsc.ldt;
+  const int32_t rmax = dsc.rmax, cmax = dsc.cmax;
   const int32_t ncols = cmax - c0, nrows = rmax - r0;
-  const int32_t base = kdepR[r0];
-  const int32_t lbase = lrankC[c0];
+  const int32_t base = dsc.base_rx & 0x7FFFFFFF;
+  const int32_t lbase = dsc.lbase;
   const int32_t wb = lrankC[cmax] - lbase - 1;
   const int32_t rowlen = wb + 1;
   uint16_t* slots = tslots + seg * (2 * slot_ints);
@@ -1016,6 +1005,7 @@ void launch_tiny(const DPArgs& a, int slot_bytes, hipStream_t s) {
   for (int64_t off = 0; off < a.ntasks; off += kChunk) {
     DPArgs c = a;
     c.tasks = a.tasks + off;
+    c.descs = a.descs + off;
     c.ntasks = std::min(kChunk, a.ntasks - off);
     hipLaunchKernelGGL(tiny_kernel, dim3((unsigned)((c.ntasks + kTinyPerGroup - 1) / kTinyPerGroup)),
                        dim3(256), (size_t)kTinyPerGroup * slot_ints * 4, s, c, slot_ints);
@@ -1088,6 +1078,7 @@ static void launch_row(const DPArgs& a, int lds_bytes, hipStream_t s) {
   for (int64_t off = 0; off < a.ntasks; off += kChunk) {
     DPArgs c = a;
     c.tasks = a.tasks + off;
+    c.descs = a.descs + off;
     c.ntasks = std::min(kChunk, a.ntasks - off);
     if (a.costs)
       hipLaunchKernelGGL((rowscan_kernel<W, C, false, true>), dim3((unsigned)c.ntasks),

@@ -68,6 +68,7 @@ struct PlanArgs {
   unsigned long long* cursor;  // scatter cursors (bucket offsets)
   int32_t* maxlds;             // nlevels*kNumClasses: largest slot storage per bucket
   uint64_t* tasks;
+  TaskDesc* descs;             // one per task, same order
   unsigned int* tickets;       // nlevels*kNumClasses: column-blocked launch counters
   int32_t* abort_flag;         // 8 ints: column-blocked waits gave up, and what they waited for
 };
@@ -114,6 +115,7 @@ struct DPArgs {
   const PairMeta* pairs;
   int32_t* td;            // arena holding every TD / TDT
   const uint64_t* tasks;  // this launch's tables
+  const TaskDesc* descs;  // and their descriptors
   int64_t ntasks;
   // Column-blocked launches only.
   unsigned long long* xl_words;  // cross-workgroup words of every such task

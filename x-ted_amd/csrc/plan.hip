@@ -247,6 +247,29 @@ __device__ __forceinline__ unsigned long long wave_reserve(const PlanArgs& A, Pl
   return pos;
 }
 
+// The start-up values of table (i0, j0) of a pair.
+__device__ __forceinline__ TaskDesc make_desc(const PlanArgs& A, const PairMeta& pm,
+                                              const TreeMeta& tx, const TreeMeta& ty, int32_t i0,
+                                              int32_t j0) {
+  const int32_t a = A.pool.orl[tx.node_off + i0] - i0 + 1;
+  const int32_t b = A.pool.orl[ty.node_off + j0] - j0 + 1;
+  const bool rows_x = a <= b;
+  TaskDesc d;
+  d.tdo = rows_x ? pm.td_off : pm.tdt_off;
+  d.tdt = rows_x ? pm.tdt_off : pm.td_off;
+  d.offR = rows_x ? tx.node_off : ty.node_off;
+  d.offC = rows_x ? ty.node_off : tx.node_off;
+  d.ldo = rows_x ? ty.n : tx.n;
+  d.ldt = rows_x ? tx.n : ty.n;
+  d.r0 = rows_x ? i0 : j0;
+  d.c0 = rows_x ? j0 : i0;
+  d.rmax = d.r0 + (rows_x ? a : b);
+  d.cmax = d.c0 + (rows_x ? b : a);
+  d.base_rx = A.pool.kdep[d.offR + d.r0] | (rows_x ? (int32_t)0x80000000 : 0);
+  d.lbase = A.pool.lrank[d.offC + d.c0];
+  return d;
+}
+
 template <bool kScatter>
 __global__ __launch_bounds__(kPlanThreads) void plan_kernel(PlanArgs A) {
   __shared__ PlanLds T;
@@ -297,7 +320,10 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(PlanArgs A) {
     __syncthreads();
     sweep([&](bool active, int bucket, int, uint32_t i0, uint32_t j0, unsigned long long) {
       const unsigned long long pos = wave_reserve(A, T, bucket, active);
-      if (active) A.tasks[pos] = task_pack((uint32_t)ch.p, i0, j0);
+      if (active) {
+        A.tasks[pos] = task_pack((uint32_t)ch.p, i0, j0);
+        A.descs[pos] = make_desc(A, pm, tx, ty, (int32_t)i0, (int32_t)j0);
+      }
     });
   }
 }
