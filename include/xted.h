@@ -171,8 +171,8 @@ int xted_set_graphs(xted_context* ctx, int32_t on);
  * 4096), del[a] / ins[a] for deleting / inserting a node labelled a and
  * ren[a*alphabet + b] for relabelling a (an x label) to b (a y label), each
  * in [0, 32767]. Applies to every later xted_ted, xted_batch and server
- * query of the context; every table then takes the row-scan forms (the
- * closed-form leaf tables and the tiny form assume unit costs), and a pair
+ * query of the context; the tables of a leaf keyroot then take the row-scan
+ * and tiny forms (their closed form assumes unit costs), and a pair
  * whose forest values can exceed 65535 (deleting all of x plus inserting
  * all of y) is rejected with XTED_ERR_TOO_LARGE. xted_ted of an empty tree
  * returns the cost of inserting (deleting) every node of the other.

@@ -74,7 +74,7 @@ def test_random_cost_models(cost_ctx, seed):
 
 def test_unit_tables_match_unit_mode(cost_ctx):
     """A cost model of unit tables takes the cost kernels and must reproduce
-    unit mode, which takes the leaf and tiny forms as well."""
+    unit mode, which takes the closed-form leaf tables as well."""
     x = gen(xted.GEN_DEEP, 1000, 30, 210, 0, (60, 120))
     y = gen(xted.GEN_SHALLOW, 900, 30, 210, 1)
     d0, t0, _ = cost_ctx.ted(x, y)
@@ -119,16 +119,21 @@ def test_every_column_blocked_form_forced(cost_ctx, cls):
         assert_same(cost_ctx, x, y, costs, threads=16)
 
 
-def test_tiny_class_falls_back_to_row_scan(cost_ctx):
-    """Forcing the tiny form (unit costs only) under a cost model leaves the
-    choice to the row-scan rule."""
-    rng = np.random.default_rng(5)
-    costs = rand_costs(rng, 5)
+@pytest.mark.parametrize("seed", range(4))
+def test_tiny_class_forced(cost_ctx, seed):
+    """Every table of at most 16 columns in the tiny form's cost variant (four
+    tables per wave, one per 16-lane row), including the one-row tables of
+    leaf keyroots, which the cost model plans like any other."""
+    rng = np.random.default_rng(500 + seed)
+    costs = rand_costs(rng, 5, zeros=seed % 2 == 1)
     cost_ctx.set_costs(costs)
     cost_ctx.force_class(xted.CLASS_TINY, lane_enable=False)
-    x = gen(xted.GEN_RRT, 60, 5, 500, 0)
-    y = gen(xted.GEN_SHALLOW, 50, 5, 500, 1)
-    assert_same(cost_ctx, x, y, costs)
+    for i, (kx, ky, n1, n2) in enumerate([(0, 2, 60, 50), (1, 1, 90, 120), (2, 0, 16, 16),
+                                          (0, 0, 1, 30)]):
+        x = gen_any(kx, n1, 5, 500 + seed, 2 * i)
+        y = gen_any(ky, n2, 5, 500 + seed, 2 * i + 1)
+        assert_same(cost_ctx, x, y, costs)
+        assert_same(cost_ctx, y, x, costs)
 
 
 def test_batch_against_oracle(cost_ctx):

@@ -428,8 +428,8 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   g.leaf_tables = g.leaf_cells = 0;
   int32_t nlevels = 0;
   g.maxn = 1;
-  // The leaf and tiny forms assume unit costs: with a cost model every table
-  // takes the row-scan forms.
+  // The closed-form leaf tables assume unit costs: with a cost model the
+  // tables of a leaf keyroot are planned like the others.
   const int32_t le = cost_alphabet_ > 0 ? 0 : lane_enable;
   for (int32_t i = 0; i < npairs; ++i) {
     const int32_t x = xs[i], y = ys[i];
@@ -515,9 +515,8 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   pa.lchunk_tables = plan_chunk;
   pa.rule = rule;
   pa.rule_tail = ClassRule{lds_limit, 1, 1 << 30, lds_limit, 16, rule.xl_seq_rows};
-  if (cost_alphabet_ > 0) pa.rule.tiny = pa.rule_tail.tiny = 0;
   pa.tail_level = g.tail_lv.as<uint8_t>();
-  pa.force_class = cost_alphabet_ > 0 && force_class == kClassTiny ? -1 : force_class;
+  pa.force_class = force_class;
   pa.lane_enable = le;
   pa.counts = g.counts.as<unsigned long long>();
   pa.cells = g.cells.as<unsigned long long>();

@@ -893,6 +893,10 @@ __device__ __forceinline__ int seg16_incl_min(int v) {
   return v;
 }
 
+// kCost: the cost form, with the substitutions of the row scan (Q(c) from
+// the column tree's prefix sums in place of t, the row node's vertical cost
+// in place of 1, a rename gather on tree cells).
+template <bool kCost>
 __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) {
   extern __shared__ uint16_t tslots[];
   __shared__ uint16_t sink[256];
@@ -919,6 +923,19 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
   const int32_t base = dsc.base_rx & 0x7FFFFFFF;
   const int32_t lbase = dsc.lbase;
   const int32_t wb = lrankC[cmax] - lbase - 1;
+  const bool rows_x = dsc.base_rx < 0;
+  // Cost prefix sums of the column (Hpre) and row (Vpre) trees; hq, hc, vr
+  // as in the row scan.
+  const int32_t* __restrict__ Hpre = nullptr;
+  const int32_t* __restrict__ Vpre = nullptr;
+  int32_t hq = 0, hc = 0, vr = 0;
+  if constexpr (kCost) {
+    Hpre = (rows_x ? A.pool.pins : A.pool.pdel) + offC;
+    Vpre = (rows_x ? A.pool.pdel : A.pool.pins) + offR;
+    hq = Hpre[cmax - 1];
+    hc = Hpre[cmax];
+    vr = Vpre[rmax];
+  }
   const int32_t rowlen = wb + 1;
   uint16_t* slots = tslots + seg * (2 * slot_ints);
 
@@ -928,16 +945,16 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
   const int32_t labc = labC[c];
   const int32_t ocm = orlC[max(c - 1, c0)];
   const bool tree_col = oc == cmax - 1;
-  const int32_t bfar = tree_col ? -1 : cmax - oc - 1;
+  const int32_t bfar = tree_col ? -1 : kCost ? hc - Hpre[oc + 1] : cmax - oc - 1;
   const int32_t foff = tree_col ? wb : lrankC[oc] - lbase;
   const bool col_ok = live && t < ncols;
   const bool writes = col_ok && c > c0 && ocm == c - 1;
   uint16_t* const wdst_rel = writes ? slots + (lrankC[max(c - 1, c0)] - lbase) : sink + threadIdx.x;
-  int32_t negt = -t;
+  int32_t negt = kCost ? Hpre[c] - hq : -t;
   asm("" : "+v"(negt));
 
   // Lane t's metadata: row rmax-1-t (rows past the table read row r0).
-  int32_t mlab, mrf, mslot;
+  int32_t mlab, mrf, mslot, mvp = 0;
   {
     const int32_t r = max(rmax - 1 - t, r0);
     const int32_t rp = max(r - 1, r0);
@@ -948,6 +965,7 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
     mlab = labR[r];
     mrf = orr + 1;
     mslot = fs | (saved ? (((kdp - base - 1) << 16) | (int32_t)0x80000000) : 0);
+    if constexpr (kCost) mvp = Vpre[r];
   }
   // TD of this lane's column for every row (row k at r = rmax-1-k).
   int32_t tdv[kTinySeg];
@@ -960,8 +978,10 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
       max(max(__builtin_amdgcn_readlane(nrows, 0), __builtin_amdgcn_readlane(nrows, 16)),
           max(__builtin_amdgcn_readlane(nrows, 32), __builtin_amdgcn_readlane(nrows, 48)));
 
-  int32_t Fpm = 2;          // F[r+1][c] + 1 - t
+  // F[r+1][c] + (vertical cost of row r) - Q(c): 2 with unit costs.
+  int32_t Fpm = kCost ? (hc - hq) + (vr - Vpre[rmax - 1]) : 2;
   int32_t left_prev = 0;    // F[r+1][cmax]
+  int32_t vp_prev = vr;     // kCost: Vpre of the previous row
   // Row k (a compile-time index: it selects the DPP broadcast lane).
   auto row = [&](auto kc) {
     constexpr int k = decltype(kc)::value;
@@ -972,27 +992,44 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
     const int32_t Rf = __builtin_amdgcn_update_dpp(0, mrf, DPP_ROW_NEWBCAST(k), 0xf, 0xf, false);
     const int32_t sl = __builtin_amdgcn_update_dpp(0, mslot, DPP_ROW_NEWBCAST(k), 0xf, 0xf, false);
     const bool tree_row = Rf == rmax;
-    // F[r+1][c+1] - t: the left neighbour's previous F (lane t-1), or the
-    // boundary F[r+1][cmax] for t = 0.
-    const int32_t fpr = __builtin_amdgcn_update_dpp(left_prev, Fpm - 1 - negt, DPP_ROW_SHR(1),
+    // Vertical costs of this row and of the next, and F[r][cmax] (unit
+    // costs: 1, 1 and rmax - r).
+    int32_t vcur = 1, vnext = 1, frc = rmax - r;
+    if constexpr (kCost) {
+      const int32_t vpr = __builtin_amdgcn_update_dpp(0, mvp, DPP_ROW_NEWBCAST(k), 0xf, 0xf, false);
+      const int32_t vpn = __builtin_amdgcn_update_dpp(
+          0, mvp, DPP_ROW_NEWBCAST(k + 1 < kTinySeg ? k + 1 : k), 0xf, 0xf, false);
+      vcur = vp_prev - vpr;
+      vnext = vpr - vpn;
+      frc = vr - vpr;
+      vp_prev = vpr;
+    }
+    // F[r+1][c+1] - Q(c+1): the left neighbour's previous F (lane t-1), or
+    // the boundary F[r+1][cmax] for t = 0.
+    const int32_t fpr = __builtin_amdgcn_update_dpp(left_prev, Fpm - vcur - negt, DPP_ROW_SHR(1),
                                                     0xf, 0xf, false);
-    const int32_t diag = fpr + negt + (int32_t)(labr != labc);
+    int32_t rename;
+    if constexpr (kCost)
+      rename = A.ren[rows_x ? labr * A.alphabet + labc : labc * A.alphabet + labr];
+    else
+      rename = (int32_t)(labr != labc);
+    const int32_t diag = fpr + negt + rename;
     const int32_t sv = slots[(sl & 0xFFFF) * rowlen + foff];
     const int32_t far = (tree_row ? bfar : sv) + tdv[k] + negt;
     const int32_t dm = tree_row && tree_col ? diag : far;
-    const int32_t carry = rmax - r + 1;
-    const int32_t h = min(carry, seg16_incl_min(min(dm, Fpm)));  // F - t
+    const int32_t carry = kCost ? frc + (hc - hq) : rmax - r + 1;
+    const int32_t h = min(carry, seg16_incl_min(min(dm, Fpm)));  // F - Q(c)
     if (act) {
-      Fpm = h + 1;
-      left_prev = rmax - r;
+      Fpm = h + vnext;
+      left_prev = frc;
       if (tree_row && tree_col && col_ok) {
-        TDo[(int64_t)r * ldo + c] = h + t;
-        TDt[(int64_t)c * ldt + r] = h + t;
+        TDo[(int64_t)r * ldo + c] = h - negt;
+        TDt[(int64_t)c * ldt + r] = h - negt;
       }
       if (sl < 0) {
         uint16_t* wrow = slots + ((sl >> 16) & 0x7FFF) * rowlen;
-        *(writes ? wdst_rel + (wrow - slots) : wdst_rel) = (uint16_t)(h + t);
-        if (t == 0) wrow[wb] = (uint16_t)(rmax - r);
+        *(writes ? wdst_rel + (wrow - slots) : wdst_rel) = (uint16_t)(h - negt);
+        if (t == 0) wrow[wb] = (uint16_t)frc;
       }
     }
   };
@@ -1007,8 +1044,12 @@ void launch_tiny(const DPArgs& a, int slot_bytes, hipStream_t s) {
     c.tasks = a.tasks + off;
     c.descs = a.descs + off;
     c.ntasks = std::min(kChunk, a.ntasks - off);
-    hipLaunchKernelGGL(tiny_kernel, dim3((unsigned)((c.ntasks + kTinyPerGroup - 1) / kTinyPerGroup)),
-                       dim3(256), (size_t)kTinyPerGroup * slot_ints * 4, s, c, slot_ints);
+    const dim3 grid((unsigned)((c.ntasks + kTinyPerGroup - 1) / kTinyPerGroup));
+    const size_t lds = (size_t)kTinyPerGroup * slot_ints * 4;
+    if (a.costs)
+      hipLaunchKernelGGL(tiny_kernel<true>, grid, dim3(256), lds, s, c, slot_ints);
+    else
+      hipLaunchKernelGGL(tiny_kernel<false>, grid, dim3(256), lds, s, c, slot_ints);
   }
 }
 
