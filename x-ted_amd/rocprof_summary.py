@@ -33,6 +33,8 @@ def short(name):
         if len(parts) > 2 and parts[2] == "true":
             return f"xl_bw{64 * w * c}{cost}"
         return f"rowscan_w{w}_c{c}{cost}"
+    if base.startswith("tiny_kernel<"):
+        return "tiny_kernel_cost" if base == "tiny_kernel<true>" else "tiny_kernel"
     return base
 
 
