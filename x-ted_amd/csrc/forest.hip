@@ -743,7 +743,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
               int32_t fprm, rename;
               if constexpr (kCost) {
                 fprm = e ? Fpm[j - 1] - vcur - negt[j - 1] + negt[j] : fpr0[g] + negt[j];
-                rename = A.ren[rows_x ? labr * A.alphabet + labc[j] : labc[j] * A.alphabet + labr];
+                // Columns off the tree path read entry 0 (one line for the
+                // wave) instead of a row of the matrix.
+                rename = A.ren[(rows_x ? labr * A.alphabet + labc[j] : labc[j] * A.alphabet + labr) &
+                               (bfar[j] >> 31)];
               } else {
                 fprm = e ? Fpm[j - 1] - 2 : fpr0[g] + negt[j];
                 rename = (int32_t)(labr != labc[j]);
