@@ -1013,7 +1013,8 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
                                                     0xf, 0xf, false);
     int32_t rename;
     if constexpr (kCost)
-      rename = A.ren[rows_x ? labr * A.alphabet + labc : labc * A.alphabet + labr];
+      rename = A.ren[(rows_x ? labr * A.alphabet + labc : labc * A.alphabet + labr) &
+                     (tree_col ? -1 : 0)];
     else
       rename = (int32_t)(labr != labc);
     const int32_t diag = fpr + negt + rename;
