@@ -176,7 +176,10 @@ int xted_set_graphs(xted_context* ctx, int32_t on);
  * whose forest values can exceed 65535 (deleting all of x plus inserting
  * all of y) is rejected with XTED_ERR_TOO_LARGE. xted_ted of an empty tree
  * returns the cost of inserting (deleting) every node of the other.
- * del == NULL restores unit costs. Under a cost model xted_pool_add rejects
+ * del == NULL restores unit costs. The call first waits for the work the
+ * context has launched; server queries planned under the previous model and
+ * not yet fully launched then fail with XTED_ERR_INVALID_ARG. Under a cost
+ * model xted_pool_add rejects
  * a tree with a label outside the alphabet; a tree added before the model
  * stays in the pool, and a query that uses one fails with
  * XTED_ERR_INVALID_ARG. */

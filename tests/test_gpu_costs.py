@@ -257,3 +257,52 @@ def test_full_batch_under_costs_sampled(cost_ctx):
     for k in rng.choice(4096, 6, replace=False):
         od, _ = oracle.ted_costs(trees[2 * k], trees[2 * k + 1], costs, threads=16, want_td=False)
         assert dist[k] == od
+
+
+def _drain(srv, want, timeout_s=120):
+    import time
+    got = {}
+    t_end = time.time() + timeout_s
+    while len(got) < want and time.time() < t_end:
+        for tag, status, dist, lat, queue in srv.poll(timeout_ms=50):
+            got[tag] = (status, dist)
+    assert len(got) == want, f"only {len(got)} of {want} results"
+    return got
+
+
+def test_server_across_a_cost_model_change():
+    """Serving under a cost model, and a change of model while queries are in
+    flight: each of those finishes under the model it was planned with or
+    fails with XTED_ERR_INVALID_ARG (never a mixed result); queries after the
+    change use the new model."""
+    rng = np.random.default_rng(21)
+    trees = [gen_any(int(rng.integers(0, 3)), int(rng.integers(2, 300)), 10, 960, i)
+             for i in range(30)]
+    c1, c2 = rand_costs(rng, 10), rand_costs(rng, 10, hi=20)
+    qs = [(int(rng.integers(30)), int(rng.integers(30))) for _ in range(60)]
+    later = [(int(rng.integers(30)), int(rng.integers(30))) for _ in range(20)]
+    with xted.Context(0) as ctx:
+        ids = ctx.pool_add(trees)
+        ctx.set_costs(c1)
+        srv = xted.Server(ctx, hbm_budget=1 << 30, max_batch=16)
+        try:
+            for k, (a, b) in enumerate(qs):
+                srv.submit(int(ids[a]), int(ids[b]), k)
+            ctx.set_costs(c2)
+            first = _drain(srv, len(qs))
+            for k, (a, b) in enumerate(later):
+                srv.submit(int(ids[a]), int(ids[b]), 100 + k)
+            second = _drain(srv, len(later))
+        finally:
+            srv.close()
+    for k, (a, b) in enumerate(qs):
+        status, dist = first[k]
+        assert status in (xted.OK, xted.ERR_INVALID_ARG), status
+        if status == xted.OK:
+            d1 = oracle.ted_costs(trees[a], trees[b], c1, threads=4, want_td=False)[0]
+            d2 = oracle.ted_costs(trees[a], trees[b], c2, threads=4, want_td=False)[0]
+            assert dist in (d1, d2), (k, dist, d1, d2)
+    for k, (a, b) in enumerate(later):
+        status, dist = second[100 + k]
+        assert status == xted.OK
+        assert dist == oracle.ted_costs(trees[a], trees[b], c2, threads=4, want_td=False)[0]

@@ -304,7 +304,13 @@ int Context::cost_prefix(int32_t first, int32_t ntrees) {
 int Context::set_costs(int32_t alphabet, const int32_t* del, const int32_t* ins,
                        const int32_t* ren) {
   std::lock_guard<std::mutex> lk(mu_);
+  // Work already launched finishes under the old model before its buffers
+  // change; groups planned under it and not yet fully launched are refused
+  // at their next launch.
+  HIP_TRY(hipSetDevice(device_));
+  HIP_TRY(hipDeviceSynchronize());
   if (!del) {
+    ++cost_epoch_;
     cost_alphabet_ = 0;
     return OK;
   }
@@ -322,7 +328,8 @@ int Context::set_costs(int32_t alphabet, const int32_t* del, const int32_t* ins,
     set_error("set_costs: costs must lie in [0, 32767]");
     return ERR_INVALID_ARG;
   }
-  HIP_TRY(hipSetDevice(device_));
+  ++cost_epoch_;
+  cost_alphabet_ = 0;  // until the new model is in place
   ST_TRY(cdel_.grow(sizeof(int32_t) * alphabet, false, stream_));
   ST_TRY(cins_.grow(sizeof(int32_t) * alphabet, false, stream_));
   ST_TRY(cren_.grow(sizeof(int32_t) * a2, false, stream_));
@@ -586,6 +593,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   HIP_TRY(hipEventRecord(plan_done_, ps));
   HIP_TRY(hipStreamWaitEvent(lane_stream(lane), plan_done_, 0));
   g.npairs = npairs;
+  g.cost_epoch = cost_epoch_;
   g.nlevels = nlevels;
   g.next_level = 0;
   g.level_off = 0;
@@ -604,6 +612,12 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
     unsigned long long off, cnt, cells;
   };
   std::vector<Launch> lv, leaf;
+  for (int gi = 0; gi < ng; ++gi) {
+    if (gs[gi]->cost_epoch != cost_epoch_) {
+      set_error("launch: the cost model changed after these pairs were planned");
+      return ERR_INVALID_ARG;
+    }
+  }
   for (int gi = 0; gi < ng; ++gi) {
     Group& g = *gs[gi];
     const int L = g.next_level;

@@ -56,6 +56,7 @@ struct ClassProfile {
 // server keeps several in flight, each at its own level.
 struct Group {
   int32_t npairs = 0, nlevels = 0, maxn = 1;
+  uint64_t cost_epoch = 0;          // cost model the group was planned under
   int32_t next_level = 0;           // next level to launch
   unsigned long long level_off = 0; // tasks of the levels already launched
   int64_t nxl = 0, leaf_tables = 0, leaf_cells = 0, td_bytes = 0;
@@ -118,6 +119,7 @@ class Context {
   // row = x label). del == nullptr restores unit costs.
   int set_costs(int32_t alphabet, const int32_t* del, const int32_t* ins, const int32_t* ren);
   bool cost_mode() const { return cost_alphabet_ > 0; }
+  uint64_t cost_epoch() const { return cost_epoch_; }  // changes with every set_costs
   // Cost of deleting (del) or inserting every node of a tree under the cost
   // model: the distance between it and the empty tree. -1 for a label outside
   // the alphabet.
@@ -209,6 +211,7 @@ class Context {
   int cost_prefix(int32_t first, int32_t ntrees);  // pdel_ / pins_ and totals of those trees
   int32_t cost_alphabet_ = 0;                       // 0: unit costs
   std::vector<int32_t> hdel_, hins_;                // host copies of the per-label costs
+  uint64_t cost_epoch_ = 0;                         // bumped by every set_costs
   DevBuf cdel_, cins_, cren_, pdel_, pins_, ctot_;
   std::vector<uint64_t> graph_signature(const Group& g, int lane) const;
   int run_levels(Group& g, int lane, std::vector<ProfLaunch>* prof);

@@ -159,9 +159,23 @@ bool Server::tick(int lane) {
   }
   (void)hipGetLastError();  // a not-ready query leaves its status behind
   if (L.active.empty() || (int)L.ticks.size() >= kTicksInFlight) return false;
+  std::lock_guard<std::mutex> lk(ctx_->mutex());
+  // Groups planned under a cost model that has since changed cannot finish:
+  // their queries fail, the other groups go on.
+  for (size_t i = 0; i < L.active.size();) {
+    Active& a = L.active[i];
+    if (a.g->cost_epoch == ctx_->cost_epoch()) {
+      ++i;
+      continue;
+    }
+    publish(a.reqs, ERR_INVALID_ARG, nullptr, a.t_admit);
+    inflight_bytes_ -= a.bytes;
+    spare_.push_back(std::move(a.g));
+    L.active.erase(L.active.begin() + i);
+  }
+  if (L.active.empty()) return false;
   std::vector<Group*> gs;
   for (auto& a : L.active) gs.push_back(a.g.get());
-  std::lock_guard<std::mutex> lk(ctx_->mutex());
   const int st = ctx_->launch_levels(gs.data(), (int)gs.size(), nullptr, lane);
   hipEvent_t ev = nullptr;
   if (free_ev_.empty()) {
