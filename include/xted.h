@@ -75,7 +75,9 @@ const char* xted_status_string(int32_t status);
  * scheduler behind them, parallel_standard_ted_test
  * (X-TED_GPU/parallel_X_TED.cu:169-399). td (m*n ints) may be NULL.
  * Empty trees follow the reference: m == 0 gives n, n == 0 gives m
- * (X-TED_GPU/TED.cu:119-124). */
+ * (X-TED_GPU/TED.cu:119-124). The pair goes through the context's tree
+ * pool, which this call (and xted_edit_mapping) clears first: keep pools
+ * that batches or a server use on contexts of their own. */
 int xted_ted(xted_context* ctx, const int32_t* x_parent, const int32_t* x_label, int32_t m,
              const int32_t* y_parent, const int32_t* y_label, int32_t n, int32_t* td,
              int32_t* dist, xted_stats* stats);

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <mutex>
 #include <exception>
 #include <new>
 #include <vector>
@@ -16,6 +17,9 @@ using namespace xted;
 
 struct xted_context {
   Context ctx;
+  // Serialises the entry points on one context: xted_ted and
+  // xted_edit_mapping use the pool as scratch over several engine calls.
+  std::recursive_mutex api;
   explicit xted_context(int d) : ctx(d) {}
 };
 
@@ -98,6 +102,8 @@ int xted_ted(xted_context* ctx, const int32_t* x_parent, const int32_t* x_label,
              const int32_t* y_parent, const int32_t* y_label, int32_t n, int32_t* td,
              int32_t* dist, xted_stats* stats) {
   return guarded([&]() -> int {
+    if (!ctx) return ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> api(ctx->api);
     if (!ctx || !dist || m < 0 || n < 0) {
       set_error("xted_ted: invalid argument");
       return ERR_INVALID_ARG;
@@ -151,6 +157,8 @@ int xted_edit_mapping(xted_context* ctx, const int32_t* x_parent, const int32_t*
                       int32_t m, const int32_t* y_parent, const int32_t* y_label, int32_t n,
                       int32_t* map_x, int32_t* map_y, int32_t* nmap, int32_t* dist) {
   return guarded([&]() -> int {
+    if (!ctx) return ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> api(ctx->api);
     if (!ctx || !nmap || !dist || m < 0 || n < 0 || (m > 0 && n > 0 && (!map_x || !map_y))) {
       set_error("xted_edit_mapping: invalid argument");
       return ERR_INVALID_ARG;
@@ -220,6 +228,8 @@ int xted_mapping_from_td(const int32_t* x_parent, const int32_t* x_label, int32_
 int xted_tree_structure(xted_context* ctx, const int32_t* parent, int32_t n, int32_t* orl,
                         int32_t* kr, int32_t* K, int32_t* kr_depth) {
   return guarded([&]() -> int {
+    if (!ctx) return ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> api(ctx->api);
     if (!ctx || !parent || n <= 0) {
       set_error("xted_tree_structure: invalid argument");
       return ERR_INVALID_ARG;
@@ -238,6 +248,8 @@ int xted_tree_structure(xted_context* ctx, const int32_t* parent, int32_t n, int
 int xted_pool_add(xted_context* ctx, int32_t ntrees, const int32_t* sizes,
                   const int32_t* parents, const int32_t* labels, int32_t* ids) {
   return guarded([&]() -> int {
+    if (!ctx) return ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> api(ctx->api);
     if (!ctx) {
       set_error("xted_pool_add: null context");
       return ERR_INVALID_ARG;
@@ -248,11 +260,13 @@ int xted_pool_add(xted_context* ctx, int32_t ntrees, const int32_t* sizes,
 
 int xted_pool_clear(xted_context* ctx) {
   if (!ctx) return ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> api(ctx->api);
   return ctx->ctx.pool_clear();
 }
 
 int xted_pool_size(xted_context* ctx, int32_t* ntrees) {
   if (!ctx || !ntrees) return ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> api(ctx->api);
   *ntrees = ctx->ctx.pool_size();
   return OK;
 }
@@ -260,6 +274,8 @@ int xted_pool_size(xted_context* ctx, int32_t* ntrees) {
 int xted_batch(xted_context* ctx, int32_t npairs, const int32_t* x_ids, const int32_t* y_ids,
                int32_t* dist, int32_t* const* td_out, xted_stats* stats) {
   return guarded([&]() -> int {
+    if (!ctx) return ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> api(ctx->api);
     if (!ctx) {
       set_error("xted_batch: null context");
       return ERR_INVALID_ARG;
@@ -337,6 +353,8 @@ int xted_set_batch_groups(xted_context* ctx, int32_t groups, int32_t split_min) 
 int xted_set_costs(xted_context* ctx, int32_t alphabet, const int32_t* del, const int32_t* ins,
                    const int32_t* ren) {
   return guarded([&]() -> int {
+    if (!ctx) return ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> api(ctx->api);
     if (!ctx) return ERR_INVALID_ARG;
     return ctx->ctx.set_costs(alphabet, del, ins, ren);
   });
