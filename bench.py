@@ -16,6 +16,7 @@ The same JSON line also reports:
     HBM, timed with events around each of its launches;
   * the reference's own X-TED_CPU (compiled from its sources into
     oracle/_ref/) on a bounded sample of the same pairs, on this host;
+  * all-pairs mode: the distance matrix of the first 128 trees (8128 pairs);
   * the same batch under a non-unit cost model (per-label delete / insert
     costs and a rename matrix): pairs/s and its dominant kernel's roofline;
   * the extra-large pair (two 20,000-node random recursive trees): GCUPS
@@ -355,6 +356,21 @@ def main():
                     "roofline": croof, "classes": cclasses}
         log(f"cost model: {cost_leg}")
 
+    # All-pairs / dataset join: the distance matrix of the first 128 trees of
+    # the workload (8128 pairs, in one plan); this clears the pool.
+    ap_trees = trees[:128]
+    ctx.all_pairs(ap_trees)  # warm
+    t = time.perf_counter()
+    apd, apst = ctx.all_pairs(ap_trees)
+    ap_s = time.perf_counter() - t
+    if not (apd == apd.T).all():
+        raise SystemExit("all-pairs matrix is not symmetric")
+    npairs_ap = len(ap_trees) * (len(ap_trees) - 1) // 2
+    all_pairs = {"trees": len(ap_trees), "pairs": npairs_ap, "ms": round(ap_s * 1e3, 2),
+                 "pairs_per_s": round(npairs_ap / ap_s, 1),
+                 "gcups": round(apst.cells / ap_s / 1e9, 2)}
+    log(f"all pairs: {all_pairs}")
+
     # Plain per-pair drop-in path on the first pairs of the batch.
     n_plain = min(args.plain_pairs, args.pairs)
     t = time.perf_counter()
@@ -412,6 +428,7 @@ def main():
         "classes": classes,
         "cpu_baseline": cpu,
         "cost_model": cost_leg,
+        "all_pairs": all_pairs,
         "xl_pair": xl,
         "serving": serving,
     }

@@ -127,6 +127,18 @@ int xted_pool_size(xted_context* ctx, int32_t* ntrees);
  * m x n table. */
 int xted_batch(xted_context* ctx, int32_t npairs, const int32_t* x_ids, const int32_t* y_ids,
                int32_t* dist, int32_t* const* td_out, xted_stats* stats);
+/* All-pairs / dataset join: the ntrees x ntrees distance matrix of a set of
+ * trees, dist[i*ntrees + j] = TED(tree i, tree j) (trees concatenated as in
+ * xted_pool_add). The reference's test() sizes a dimension x dimension
+ * Distance matrix but fills only (0, 1) (X-TED_GPU/test_TED.cu:30-54). The
+ * pairs run through the batched plan in chunks whose tree-distance tables
+ * take at most max_bytes of HBM (0: 16 GiB). With unit costs the matrix is
+ * symmetric with a zero diagonal and only i < j is computed; under a cost
+ * model every ordered pair, the diagonal included, is. The context's pool
+ * is used as scratch (cleared first). stats sums the chunks. */
+int xted_all_pairs(xted_context* ctx, int32_t ntrees, const int32_t* sizes,
+                   const int32_t* parents, const int32_t* labels, int64_t max_bytes,
+                   int32_t* dist, xted_stats* stats);
 
 /* ---- tuning and test hooks ------------------------------------------------------
  * Granularity cut-overs, the role of the fixed thresholds 15 / 18,750 /

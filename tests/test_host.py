@@ -108,3 +108,19 @@ def test_text_formats_match_reference_tokenisation():
     assert xted_text.parse_adjacency("[[]]") == [[]]
     with open(os.path.join(GOLDEN, "datasets", "bolzano_nodes.txt")) as f:
         assert f.readline().endswith(" \n")
+
+
+def test_load_trees_reads_every_tree_with_shared_label_ids():
+    import xted_text
+    ids = {}
+    a = xted_text.load_trees(os.path.join(GOLDEN, "datasets", "dblp_100_nodes.txt"),
+                             os.path.join(GOLDEN, "datasets", "dblp_100_adj.txt"), ids)
+    b = xted_text.load_trees(os.path.join(GOLDEN, "datasets", "python_100_nodes.txt"),
+                             os.path.join(GOLDEN, "datasets", "python_100_adj.txt"), ids)
+    assert len(a) == 2 and len(b) == 2
+    first = xted_text.load_pair(os.path.join(GOLDEN, "datasets", "dblp_100_nodes.txt"),
+                                os.path.join(GOLDEN, "datasets", "dblp_100_adj.txt"))
+    for (p, l), (q, m) in zip(a, first):
+        assert np.array_equal(p, q) and np.array_equal(l, m)
+    used = np.concatenate([t[1] for t in a + b])
+    assert used.min() >= 0 and used.max() < len(ids)

@@ -287,6 +287,72 @@ int xted_batch(xted_context* ctx, int32_t npairs, const int32_t* x_ids, const in
   });
 }
 
+int xted_all_pairs(xted_context* ctx, int32_t ntrees, const int32_t* sizes,
+                   const int32_t* parents, const int32_t* labels, int64_t max_bytes,
+                   int32_t* dist, xted_stats* stats) {
+  return guarded([&]() -> int {
+    if (!ctx) return ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> api(ctx->api);
+    if (ntrees < 0 || (ntrees > 0 && (!sizes || !parents || !labels || !dist))) {
+      set_error("xted_all_pairs: invalid argument");
+      return ERR_INVALID_ARG;
+    }
+    if (stats) *stats = xted_stats{};
+    if (ntrees == 0) return OK;
+    if (max_bytes <= 0) max_bytes = int64_t{16} << 30;
+    Context& c = ctx->ctx;
+    int st = c.pool_clear();
+    if (st != OK) return st;
+    std::vector<int32_t> ids(ntrees);
+    st = c.pool_add(ntrees, sizes, parents, labels, ids.data());
+    if (st != OK) return st;
+    const bool sym = !c.cost_mode();
+    const int64_t nn = (int64_t)ntrees * ntrees;
+    std::fill(dist, dist + nn, 0);
+    // Pairs in row order, cut into chunks by their TD + TD^T bytes.
+    std::vector<int32_t> xs, ys, out;
+    int64_t bytes = 0;
+    auto flush = [&]() -> int {
+      if (xs.empty()) return OK;
+      out.resize(xs.size());
+      BatchStats bs;
+      const int r = c.run_batch((int32_t)xs.size(), xs.data(), ys.data(), out.data(), nullptr, &bs);
+      if (r != OK) return r;
+      for (size_t k = 0; k < xs.size(); ++k) {
+        const int32_t i = xs[k] - ids[0], j = ys[k] - ids[0];
+        dist[(int64_t)i * ntrees + j] = out[k];
+        if (sym) dist[(int64_t)j * ntrees + i] = out[k];
+      }
+      if (stats) {
+        stats->plan_ms += bs.plan_ms;
+        stats->dp_ms += bs.dp_ms;
+        stats->total_ms += bs.total_ms;
+        stats->cells += bs.cells;
+        stats->tables += bs.tables;
+        stats->levels = std::max<int32_t>(stats->levels, bs.levels);
+        stats->launches += bs.launches;
+      }
+      xs.clear();
+      ys.clear();
+      bytes = 0;
+      return OK;
+    };
+    for (int32_t i = 0; i < ntrees; ++i) {
+      for (int32_t j = sym ? i + 1 : 0; j < ntrees; ++j) {
+        const int64_t b = 8 * (int64_t)sizes[i] * sizes[j] + 64;
+        if (!xs.empty() && (bytes + b > max_bytes || xs.size() >= (size_t)(1 << 20))) {
+          st = flush();
+          if (st != OK) return st;
+        }
+        xs.push_back(ids[i]);
+        ys.push_back(ids[j]);
+        bytes += b;
+      }
+    }
+    return flush();
+  });
+}
+
 int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t max_cpl,
                     int32_t min_waves, int32_t wide_cols) {
   const bool pow2 = max_cpl == 1 || max_cpl == 2 || max_cpl == 4 || max_cpl == 8 || max_cpl == 16;

@@ -78,6 +78,8 @@ _SIGNATURES = {
     "xted_set_level_streams": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_batch_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "xted_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "xted_all_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32P, _I32P, _I32P,
+                                      ctypes.c_int64, _I32P, ctypes.c_void_p]),
     "xted_set_costs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32P, _I32P, _I32P]),
     "xted_set_tail_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_xl_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
@@ -303,6 +305,21 @@ class Context:
                                 ctypes.cast(arr, ctypes.c_void_p) if arr is not None else None,
                                 ctypes.byref(st)))
         return dist, tds, st
+
+    def all_pairs(self, trees, max_bytes=0):
+        """The len(trees) x len(trees) distance matrix of trees ((parent,
+        label) pairs) and the summed Stats; the pool is used as scratch."""
+        n = len(trees)
+        dist = np.zeros((n, n), np.int32)
+        st = Stats()
+        if n == 0:
+            return dist, st
+        sizes = np.array([len(t[0]) for t in trees], np.int32)
+        parents = np.ascontiguousarray(np.concatenate([np.asarray(t[0], np.int32) for t in trees]))
+        labels = np.ascontiguousarray(np.concatenate([np.asarray(t[1], np.int32) for t in trees]))
+        _check(lib().xted_all_pairs(self._h, n, _p(sizes), _p(parents), _p(labels), max_bytes,
+                                    _p(dist), ctypes.byref(st)))
+        return dist, st
 
     def set_tuning(self, lds_per_wave=10 * 1024, max_cpl=8, min_waves=1, wide_cols=1 << 30):
         _check(lib().xted_set_tuning(self._h, lds_per_wave, max_cpl, min_waves, wide_cols))

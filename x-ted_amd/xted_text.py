@@ -49,6 +49,28 @@ def adj_to_parent(adj):
     return parent
 
 
+def load_trees(labels_path, adj_path, ids=None):
+    """Every tree of a label file and its adjacency file, labels interned
+    through `ids` (a dict shared across files, so equal strings get equal
+    ids). Blank trailing lines are skipped."""
+    if ids is None:
+        ids = {}
+    with open(labels_path) as f:
+        llines = f.read().split("\n")
+    with open(adj_path) as f:
+        alines = f.read().split("\n")
+    trees = []
+    for lline, aline in zip(llines, alines):
+        if not aline.strip():
+            continue
+        adj = parse_adjacency(aline)
+        parent = adj_to_parent(adj)
+        labels = split_labels(lline)
+        lab = np.array([ids.setdefault(s, len(ids)) for s in labels[:len(adj)]], np.int32)
+        trees.append((parent, lab))
+    return trees
+
+
 def load_pair(labels_path, adj_path, first=0):
     with open(labels_path) as f:
         llines = f.read().split("\n")
