@@ -234,6 +234,22 @@ int xted_server_destroy(xted_server* srv);
 /* ---- host utilities (no GPU needed) ---------------------------------------------------
  * Seeded synthetic trees: kind 0 = random recursive tree, 1 = deep/narrow,
  * 2 = shallow/wide; depth_min/depth_max > 0 reject trees outside the band. */
+/* Text formats (host only). Bracket notation, one tree per line as
+ * {label{child}{child}...}: the reference ships such files
+ * (Sampled_Dataset_Extra_Large_Trees/synthetic/<n>_nodes/synthetic_tree_<n>.txt)
+ * but reads none of them. xted_bracket_to_lines converts one tree to the
+ * reference's input lines (README.md:31-42): the label line (each label
+ * followed by a space) and the preorder adjacency line ("[[1, 2], [], []]").
+ * xted_lines_to_bracket converts back. Outputs are NUL-terminated; *_need
+ * receives the bytes each needs (with the NUL) and a short buffer gives
+ * XTED_ERR_TOO_LARGE with nothing written; malformed text gives
+ * XTED_ERR_PARSE. No counterpart in the reference. */
+int xted_bracket_to_lines(const char* bracket, char* labels_line, int64_t labels_cap,
+                          char* adj_line, int64_t adj_cap, int64_t* labels_need,
+                          int64_t* adj_need);
+int xted_lines_to_bracket(const char* labels_line, const char* adj_line, char* bracket,
+                          int64_t cap, int64_t* need);
+
 int xted_generate(int32_t kind, int32_t n, int32_t alphabet, uint64_t seed, int32_t depth_min,
                   int32_t depth_max, int32_t* parent, int32_t* label);
 uint64_t xted_derive_seed(uint64_t config, uint64_t index);

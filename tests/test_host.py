@@ -124,3 +124,48 @@ def test_load_trees_reads_every_tree_with_shared_label_ids():
         assert np.array_equal(p, q) and np.array_equal(l, m)
     used = np.concatenate([t[1] for t in a + b])
     assert used.min() >= 0 and used.max() < len(ids)
+
+
+def _synthetic_files():
+    d = os.path.join(GOLDEN, "datasets")
+    with open(os.path.join(d, "xl_synthetic_1000_tree.txt")) as f:
+        brackets = [l for l in f.read().split("\n") if l.strip()]
+    with open(os.path.join(d, "xl_synthetic_1000_nodes.txt")) as f:
+        nodes = f.read().split("\n")
+    with open(os.path.join(d, "xl_synthetic_1000_adj.txt")) as f:
+        adjs = f.read().split("\n")
+    return brackets, nodes, adjs
+
+
+def test_bracket_notation_matches_the_reference_files():
+    """The reference ships each synthetic dataset three ways: bracket
+    notation, label lines and adjacency lines (the same two 1000-node trees);
+    converting its bracket file gives its other two files byte for byte, and
+    converting those back gives the bracket lines."""
+    brackets, nodes, adjs = _synthetic_files()
+    assert len(brackets) == 2
+    for k, b in enumerate(brackets):
+        labels_line, adj_line = xted.bracket_to_lines(b)
+        assert labels_line == nodes[k]
+        assert adj_line == adjs[k]
+        assert xted.lines_to_bracket(nodes[k], adjs[k]) == b
+
+
+def test_bracket_round_trips_and_rejects_malformed_text():
+    for b in ["{a}", "{a{b}{c{d}}}", "{x{x{x}}{y}}", "{root{c1{g1}{g2}}{c2}}"]:
+        assert xted.lines_to_bracket(*xted.bracket_to_lines(b)) == b
+    assert xted.bracket_to_lines("{a{b}{c}}") == ("a b c ", "[[1, 2], [], []]")
+    for bad in ["", "{a", "a}", "{a}{b}", "{a}}", "{a{b}} x"]:
+        with pytest.raises(xted.XtedError):
+            xted.bracket_to_lines(bad)
+
+
+def test_command_line_bracket_conversion(tmp_path):
+    brackets, nodes, adjs = _synthetic_files()
+    cli = os.path.join(ROOT, "x-ted_amd", "bin", "X-TED_GPU")
+    out_n, out_a = tmp_path / "n.txt", tmp_path / "a.txt"
+    subprocess.run([cli, "--bracket-to-lines",
+                    os.path.join(GOLDEN, "datasets", "xl_synthetic_1000_tree.txt"),
+                    str(out_n), str(out_a)], check=True, timeout=60)
+    assert out_n.read_text() == "\n".join(nodes)
+    assert out_a.read_text() == "\n".join(adjs)

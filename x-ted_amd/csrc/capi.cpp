@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstring>
 #include <mutex>
 #include <exception>
 #include <new>
@@ -350,6 +351,71 @@ int xted_all_pairs(xted_context* ctx, int32_t ntrees, const int32_t* sizes,
       }
     }
     return flush();
+  });
+}
+
+namespace {
+// Copies s with its NUL into out when it fits; *need gets the size.
+int put_text(const std::string& s, char* out, int64_t cap, int64_t* need) {
+  const int64_t n = (int64_t)s.size() + 1;
+  if (need) *need = n;
+  if (!out || cap < n) return ERR_TOO_LARGE;
+  std::memcpy(out, s.c_str(), (size_t)n);
+  return OK;
+}
+}  // namespace
+
+int xted_bracket_to_lines(const char* bracket, char* labels_line, int64_t labels_cap,
+                          char* adj_line, int64_t adj_cap, int64_t* labels_need,
+                          int64_t* adj_need) {
+  return guarded([&]() -> int {
+    if (!bracket) return ERR_INVALID_ARG;
+    std::vector<std::string> labels;
+    std::vector<int32_t> parent;
+    if (parse_bracket(bracket, labels, parent) != OK) {
+      set_error("xted_bracket_to_lines: not a bracket-notation tree");
+      return ERR_PARSE;
+    }
+    const std::string ls = format_labels(labels);
+    const std::string as =
+        format_adjacency(parent_to_adj(parent.data(), (int32_t)parent.size()));
+    if (labels_need) *labels_need = (int64_t)ls.size() + 1;
+    if (adj_need) *adj_need = (int64_t)as.size() + 1;
+    if (!labels_line || !adj_line || labels_cap < (int64_t)ls.size() + 1 ||
+        adj_cap < (int64_t)as.size() + 1) {
+      set_error("xted_bracket_to_lines: output buffer too small");
+      return ERR_TOO_LARGE;
+    }
+    put_text(ls, labels_line, labels_cap, nullptr);
+    put_text(as, adj_line, adj_cap, nullptr);
+    return OK;
+  });
+}
+
+int xted_lines_to_bracket(const char* labels_line, const char* adj_line, char* bracket,
+                          int64_t cap, int64_t* need) {
+  return guarded([&]() -> int {
+    if (!labels_line || !adj_line) return ERR_INVALID_ARG;
+    std::vector<std::vector<int>> adj;
+    std::vector<int32_t> parent;
+    const std::vector<std::string> labels = split_labels(labels_line);
+    if (parse_adjacency(adj_line, adj) != OK || adj_to_parent(adj, parent) != OK ||
+        labels.size() < adj.size() || adj.empty()) {
+      set_error("xted_lines_to_bracket: malformed label or adjacency line");
+      return ERR_PARSE;
+    }
+    for (size_t i = 0; i < adj.size(); ++i) {
+      if (labels[i].find_first_of("{} ") != std::string::npos) {
+        set_error("xted_lines_to_bracket: a label holds a brace");
+        return ERR_PARSE;
+      }
+    }
+    const std::string s = format_bracket(labels, parent.data(), (int32_t)adj.size());
+    if (put_text(s, bracket, cap, need) != OK) {
+      set_error("xted_lines_to_bracket: output buffer too small");
+      return ERR_TOO_LARGE;
+    }
+    return OK;
   });
 }
 

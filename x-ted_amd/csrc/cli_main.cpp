@@ -4,6 +4,9 @@
 // X-TED_GPU <labels_file> <adjacency_file> --all-pairs: the distance matrix
 // of every tree in the files (the dimension x dimension matrix the
 // reference's test() sizes, X-TED_GPU/test_TED.cu:30-54), one row per line.
+// X-TED_GPU --bracket-to-lines <trees_file> <labels_out> <adjacency_out>:
+// converts a bracket-notation file (one tree per line) to the two input
+// files (host only).
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -69,11 +72,46 @@ int all_pairs(const char* labels_path, const char* adj_path) {
   return 0;
 }
 
+int bracket_to_lines(const char* in_path, const char* labels_path, const char* adj_path) {
+  const vector<string> trees = read_lines(in_path);
+  std::ofstream lo(labels_path), ao(adj_path);
+  if (!lo || !ao) throw std::invalid_argument("cannot write the output files");
+  for (const string& t : trees) {
+    if (t.find('{') == string::npos) continue;  // blank line
+    int64_t ln = 0, an = 0;
+    int st = xted_bracket_to_lines(t.c_str(), nullptr, 0, nullptr, 0, &ln, &an);
+    if (st != XTED_OK && st != XTED_ERR_TOO_LARGE) {
+      std::fprintf(stderr, "%s\n", xted_last_error());
+      return 1;
+    }
+    vector<char> lb((size_t)ln), ab((size_t)an);
+    st = xted_bracket_to_lines(t.c_str(), lb.data(), ln, ab.data(), an, &ln, &an);
+    if (st != XTED_OK) {
+      std::fprintf(stderr, "%s\n", xted_last_error());
+      return 1;
+    }
+    lo << lb.data() << '\n';
+    ao << ab.data() << '\n';
+  }
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char* argv[]) {
+  if (argc >= 5 && std::strcmp(argv[1], "--bracket-to-lines") == 0) {
+    try {
+      return bracket_to_lines(argv[2], argv[3], argv[4]);
+    } catch (const std::exception& e) {
+      std::fprintf(stderr, "%s\n", e.what());
+      return 1;
+    }
+  }
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s <labels_file> <adjacency_file> [--all-pairs]\n", argv[0]);
+    std::fprintf(stderr,
+                 "usage: %s <labels_file> <adjacency_file> [--all-pairs]\n"
+                 "       %s --bracket-to-lines <trees_file> <labels_out> <adjacency_out>\n",
+                 argv[0], argv[0]);
     return 2;
   }
   try {

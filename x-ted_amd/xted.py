@@ -78,6 +78,12 @@ _SIGNATURES = {
     "xted_set_level_streams": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_batch_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "xted_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "xted_bracket_to_lines": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64,
+                                             ctypes.c_char_p, ctypes.c_int64,
+                                             ctypes.POINTER(ctypes.c_int64),
+                                             ctypes.POINTER(ctypes.c_int64)]),
+    "xted_lines_to_bracket": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                             ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "xted_all_pairs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32P, _I32P, _I32P,
                                       ctypes.c_int64, _I32P, ctypes.c_void_p]),
     "xted_set_costs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32P, _I32P, _I32P]),
@@ -368,6 +374,32 @@ class Context:
 
     def force_class(self, cls=CLASS_AUTO, lane_enable=True):
         _check(lib().xted_force_class(self._h, cls, 1 if lane_enable else 0))
+
+
+def bracket_to_lines(bracket):
+    """One bracket-notation tree -> (label line, adjacency line) in the
+    reference's input format."""
+    ln, an = ctypes.c_int64(), ctypes.c_int64()
+    src = bracket.encode()
+    st = lib().xted_bracket_to_lines(src, None, 0, None, 0, ctypes.byref(ln), ctypes.byref(an))
+    if st not in (OK, ERR_TOO_LARGE):
+        _check(st)
+    lbuf, abuf = ctypes.create_string_buffer(ln.value), ctypes.create_string_buffer(an.value)
+    _check(lib().xted_bracket_to_lines(src, lbuf, ln.value, abuf, an.value, ctypes.byref(ln),
+                                       ctypes.byref(an)))
+    return lbuf.value.decode(), abuf.value.decode()
+
+
+def lines_to_bracket(labels_line, adj_line):
+    """The reference's label and adjacency lines of one tree -> bracket notation."""
+    need = ctypes.c_int64()
+    lsrc, asrc = labels_line.encode(), adj_line.encode()
+    st = lib().xted_lines_to_bracket(lsrc, asrc, None, 0, ctypes.byref(need))
+    if st not in (OK, ERR_TOO_LARGE):
+        _check(st)
+    buf = ctypes.create_string_buffer(need.value)
+    _check(lib().xted_lines_to_bracket(lsrc, asrc, buf, need.value, ctypes.byref(need)))
+    return buf.value.decode()
 
 
 class Server:
