@@ -258,6 +258,8 @@ def main():
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--xl-only", action="store_true",
                     help="only the 20,000-node pair (for profiling it alone)")
+    ap.add_argument("--no-all-pairs", action="store_true",
+                    help="skip the all-pairs matrix of 128 trees")
     ap.add_argument("--no-costs", action="store_true",
                     help="skip the batch under a non-unit cost model")
     ap.add_argument("--serve-only", action="store_true",
@@ -358,18 +360,20 @@ def main():
 
     # All-pairs / dataset join: the distance matrix of the first 128 trees of
     # the workload (8128 pairs, in one plan); this clears the pool.
-    ap_trees = trees[:128]
-    ctx.all_pairs(ap_trees)  # warm
-    t = time.perf_counter()
-    apd, apst = ctx.all_pairs(ap_trees)
-    ap_s = time.perf_counter() - t
-    if not (apd == apd.T).all():
-        raise SystemExit("all-pairs matrix is not symmetric")
-    npairs_ap = len(ap_trees) * (len(ap_trees) - 1) // 2
-    all_pairs = {"trees": len(ap_trees), "pairs": npairs_ap, "ms": round(ap_s * 1e3, 2),
-                 "pairs_per_s": round(npairs_ap / ap_s, 1),
-                 "gcups": round(apst.cells / ap_s / 1e9, 2)}
-    log(f"all pairs: {all_pairs}")
+    all_pairs = None
+    if not args.no_all_pairs:
+        ap_trees = trees[:128]
+        ctx.all_pairs(ap_trees)  # warm
+        t = time.perf_counter()
+        apd, apst = ctx.all_pairs(ap_trees)
+        ap_s = time.perf_counter() - t
+        if not (apd == apd.T).all():
+            raise SystemExit("all-pairs matrix is not symmetric")
+        npairs_ap = len(ap_trees) * (len(ap_trees) - 1) // 2
+        all_pairs = {"trees": len(ap_trees), "pairs": npairs_ap, "ms": round(ap_s * 1e3, 2),
+                     "pairs_per_s": round(npairs_ap / ap_s, 1),
+                     "gcups": round(apst.cells / ap_s / 1e9, 2)}
+        log(f"all pairs: {all_pairs}")
 
     # Plain per-pair drop-in path on the first pairs of the batch.
     n_plain = min(args.plain_pairs, args.pairs)
