@@ -43,12 +43,13 @@ import xted  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = 4
-CONFIG_BATCH = 3
-CONFIG_SERVE = 5
-CONFIG_XL = 4
+# Seed streams of the synthetic workloads (xted.derive_seed(stream, tree index)).
+SEED_BATCH = 3
+SEED_SERVE = 5
+SEED_XL = 4
 
 
-def batch_trees(npairs, cfg=CONFIG_BATCH):
+def batch_trees(npairs, cfg=SEED_BATCH):
     """Trees 2i and 2i+1 form pair i."""
     trees = []
     for t in range(2 * npairs):
@@ -59,7 +60,7 @@ def batch_trees(npairs, cfg=CONFIG_BATCH):
     return trees
 
 
-def serving_trees(ntrees, cfg=CONFIG_SERVE):
+def serving_trees(ntrees, cfg=SEED_SERVE):
     """Pool for serving: n log-uniform in [50, 5000], deep/shallow 50/50,
     alphabet 200, seeded per tree."""
     trees = []
@@ -130,8 +131,8 @@ def xl_pair_bench(reps=3):
     """The extra-large pair (two 20,000-node random recursive trees, alphabet
     500): its widest tables take the column-blocked form. Distance only
     (the 1.6 GB table stays in HBM); median of `reps` runs."""
-    x = xted.generate(xted.GEN_RRT, 20000, 500, xted.derive_seed(CONFIG_XL, 0))
-    y = xted.generate(xted.GEN_RRT, 20000, 500, xted.derive_seed(CONFIG_XL, 1))
+    x = xted.generate(xted.GEN_RRT, 20000, 500, xted.derive_seed(SEED_XL, 0))
+    y = xted.generate(xted.GEN_RRT, 20000, 500, xted.derive_seed(SEED_XL, 1))
     with xted.Context(0) as ctx:
         ids = ctx.pool_add([x, y])
         xs, ys = ids[:1].copy(), ids[1:].copy()
@@ -277,8 +278,8 @@ def main():
                                                  budget=32 << 30, max_batch=4096)}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(pairs would shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path "
+              "(pairs would shard across GPUs by pair or by block of the all-pairs matrix, with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     t0 = time.perf_counter()

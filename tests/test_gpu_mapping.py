@@ -11,8 +11,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("kx,ky,n,alphabet", [
-    (xted.GEN_RRT, xted.GEN_RRT, 100, 50),          # config 1 shape
-    (xted.GEN_DEEP, xted.GEN_SHALLOW, 1000, 200),   # config 2 shapes
+    (xted.GEN_RRT, xted.GEN_RRT, 100, 50),          # 100-node random recursive trees
+    (xted.GEN_DEEP, xted.GEN_SHALLOW, 1000, 200),   # 1000-node deep/narrow against shallow/wide
     (xted.GEN_SHALLOW, xted.GEN_DEEP, 700, 3),      # many matches
 ])
 def test_edit_mapping_optimal(gpu_ctx, kx, ky, n, alphabet):

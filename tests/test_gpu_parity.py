@@ -35,7 +35,7 @@ def test_golden_reference_tables(gpu_ctx, case):
     assert (t == td).all()
 
 
-def test_config1_rrt100(gpu_ctx):
+def test_rrt_100_node_pairs(gpu_ctx):
     """100-node random recursive trees, alphabet 50: distance + 100x100 table."""
     for i in range(12):
         x = gen(xted.GEN_RRT, 100, 50, 1, 2 * i)
@@ -45,14 +45,14 @@ def test_config1_rrt100(gpu_ctx):
 
 @pytest.mark.parametrize("kind", [xted.GEN_DEEP, xted.GEN_SHALLOW])
 @pytest.mark.parametrize("alphabet", [200, 1000])
-def test_config2_1000_nodes(gpu_ctx, kind, alphabet):
+def test_1000_node_pairs(gpu_ctx, kind, alphabet):
     band = (60, 120) if kind == xted.GEN_DEEP else (0, 0)
     x = gen(kind, 1000, alphabet, 2, 0, band)
     y = gen(kind, 1000, alphabet, 2, 1, band)
     assert_same(gpu_ctx, x, y, threads=16)
 
 
-def test_config2_mixed_shapes(gpu_ctx):
+def test_1000_node_mixed_shapes(gpu_ctx):
     x = gen(xted.GEN_DEEP, 1000, 200, 2, 10, (60, 120))
     y = gen(xted.GEN_SHALLOW, 1000, 200, 2, 11)
     assert_same(gpu_ctx, x, y, threads=16)
