@@ -512,6 +512,24 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   for (hipEvent_t* e : {&g.ev_plan0, &g.ev_plan1, &g.ev_end})
     if (!*e) HIP_TRY(hipEventCreate(e));
   HIP_TRY(hipEventRecord(g.ev_plan0, ps));
+  // The leaf lines read only the trees and the arena layout, which are on the
+  // device now: a batch's run on its level stream while the plan is built
+  // (not when every launch is timed on its own).
+  g.leaf_early = false;
+  if (lane < 0 && !profile && g.lines[npairs] > 0) {
+    hipStream_t ls = lane_stream(lane);
+    HIP_TRY(hipStreamWaitEvent(ls, g.ev_plan0, 0));
+    LeafArgs la;
+    la.pool = view();
+    la.pairs = g.pairs.as<PairMeta>();
+    la.tile_start = g.lines_d.as<int64_t>();
+    la.npairs = npairs;
+    la.td = g.td.as<int32_t>();
+    launch_leaf(la, g.lines[npairs], ls);
+    HIP_TRY(hipGetLastError());
+    ++st.launches;
+    g.leaf_early = true;
+  }
 
   PlanArgs pa;
   pa.pool = view();
@@ -624,7 +642,7 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
     if (L >= g.nlevels) continue;
     // The leaf lines read nothing; the leaf kernel also fills cells of later
     // tables with placeholders, so it runs before the group's first level.
-    if (L == 0 && g.lines[g.npairs] > 0)
+    if (L == 0 && g.lines[g.npairs] > 0 && !g.leaf_early)
       leaf.push_back({&g, kClassLane, -1, 0, (unsigned long long)g.lines[g.npairs],
                       (unsigned long long)g.leaf_cells});
     unsigned long long off = g.level_off;
@@ -784,6 +802,7 @@ std::vector<uint64_t> Context::graph_signature(const Group& g, int lane) const {
   for (int32_t m : g.h_maxlds) s.push_back((uint64_t)(uint32_t)m);
   s.push_back((uint64_t)cost_alphabet_);
   s.push_back((uint64_t)(uintptr_t)cren_.p);
+  s.push_back((uint64_t)g.leaf_early);
   return s;
 }
 

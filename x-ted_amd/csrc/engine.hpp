@@ -57,6 +57,7 @@ struct ClassProfile {
 struct Group {
   int32_t npairs = 0, nlevels = 0, maxn = 1;
   uint64_t cost_epoch = 0;          // cost model the group was planned under
+  bool leaf_early = false;          // the leaf lines were launched while planning
   int32_t next_level = 0;           // next level to launch
   unsigned long long level_off = 0; // tasks of the levels already launched
   int64_t nxl = 0, leaf_tables = 0, leaf_cells = 0, td_bytes = 0;
