@@ -73,16 +73,24 @@ def serving_trees(ntrees, cfg=SEED_SERVE):
     return trees
 
 
+def serving_stream(ntrees, seconds, rate, seed=7):
+    """Seeded Poisson query stream over a pool of ntrees trees: arrival
+    times (s, sorted) and the x / y pool indices of uniformly random pairs."""
+    rng = np.random.default_rng(seed)
+    n_arr = max(1, int(rng.poisson(rate * seconds)))
+    at = np.sort(rng.uniform(0.0, seconds, n_arr))
+    qx = rng.integers(0, ntrees, n_arr)
+    qy = rng.integers(0, ntrees, n_arr)
+    return at, qx, qy
+
+
 def serve_bench(trees, seconds, rate, budget, max_batch, seed=7):
     """Poisson arrivals of uniformly random pairs at `rate` per second for
     `seconds`, through the continuous-batching server. Returns the latency
     percentiles (submit -> result, measured by the server), the completed
     pairs/s and a check of a sample of results against one-pair batches."""
-    rng = np.random.default_rng(seed)
-    n_arr = max(1, int(rng.poisson(rate * seconds)))
-    at = np.sort(rng.uniform(0.0, seconds, n_arr))
-    qx = rng.integers(0, len(trees), n_arr)
-    qy = rng.integers(0, len(trees), n_arr)
+    at, qx, qy = serving_stream(len(trees), seconds, rate, seed)
+    n_arr = len(at)
     ctx = xted.Context(0)
     t = time.perf_counter()
     ids = ctx.pool_add(trees)
@@ -105,13 +113,25 @@ def serve_bench(trees, seconds, rate, budget, max_batch, seed=7):
     ok = [v for v in res.values() if v[0] == 0]
     lat = np.array([v[2] for v in ok])
     span = max(v[4] for v in res.values()) if res else seconds
-    # Spot check: a sample of served results against one-pair batches.
-    sample = sorted(res)[:: max(1, len(res) // 16)]
-    for tag in sample:
-        if res[tag][0] == 0:
-            d = ctx.batch(np.array([ids[qx[tag]]]), np.array([ids[qy[tag]]]))[0][0]
-            if d != res[tag][1]:
-                raise SystemExit(f"served distance differs from the one-pair batch (query {tag})")
+    # Every served result against the oracle's distance when this is the
+    # stream the committed fixture covers; otherwise a sample against
+    # one-pair batches.
+    gold = golden_serving(seconds, rate, seed)
+    if gold is not None and len(gold) == n_arr and len(trees) == 10000:
+        for tag, v in res.items():
+            if v[0] == 0 and v[1] != gold[tag]:
+                raise SystemExit(f"served distance differs from the oracle (query {tag})")
+        checked = {"against": "oracle (tests/golden/serving_dist.npz)",
+                   "queries": sum(1 for v in res.values() if v[0] == 0)}
+    else:
+        sample = sorted(res)[:: max(1, len(res) // 16)]
+        for tag in sample:
+            if res[tag][0] == 0:
+                d = ctx.batch(np.array([ids[qx[tag]]]), np.array([ids[qy[tag]]]))[0][0]
+                if d != res[tag][1]:
+                    raise SystemExit(f"served distance differs from the one-pair batch "
+                                     f"(query {tag})")
+        checked = {"against": "one-pair batches", "queries": len(sample)}
     ctx.close()
     return {
         "offered_rate": rate, "seconds": seconds, "queries": n_arr,
@@ -123,8 +143,29 @@ def serve_bench(trees, seconds, rate, budget, max_batch, seed=7):
         "p99_ms": round(float(np.percentile(lat, 99)), 2) if len(lat) else None,
         "mean_queue_ms": round(float(np.mean([v[3] for v in ok])), 2) if ok else None,
         "hbm_budget": budget, "max_batch": max_batch, "pool_trees": len(trees),
-        "pool_upload_s": round(upload_s, 2), "checked_against_single": len(sample),
+        "pool_upload_s": round(upload_s, 2), "checked": checked,
     }
+
+
+def golden_serving(seconds, rate, seed):
+    """The oracle's distances for the serving stream (seconds, rate, seed)
+    from tests/golden/serving_dist.npz, or None for another stream."""
+    path = os.path.join(ROOT, "tests", "golden", "serving_dist.npz")
+    if not os.path.exists(path):
+        return None
+    g = np.load(path)
+    if [float(v) for v in g["stream"]] != [float(seconds), float(rate), float(seed)]:
+        return None
+    return g["dist"]
+
+
+def golden_batch(npairs):
+    """The oracle's distances for the 4096-pair batch
+    (tests/golden/batch4096_dist.npy), or None for another size."""
+    path = os.path.join(ROOT, "tests", "golden", "batch4096_dist.npy")
+    if npairs != 4096 or not os.path.exists(path):
+        return None
+    return np.load(path)
 
 
 def xl_pair_bench(reps=3):
@@ -144,34 +185,70 @@ def xl_pair_bench(reps=3):
             d, _, st = ctx.batch(xs, ys)
             runs.append((time.perf_counter() - t, st, int(d[0])))
         prof = ctx.class_profile()
+        launches = ctx.launch_profile()
     wall, st, dist = sorted(runs, key=lambda r: r[0])[len(runs) // 2]
     xl = {xted.class_name(c): round(v[3], 2) for c, v in prof.items()
           if 27 <= c < 27 + len(xted.XL_BW)}
+    classes = {xted.class_name(c): {"launches": v[0], "tables": v[1], "cells": v[2],
+                                    "ms": round(v[3], 3)} for c, v in sorted(prof.items())}
+    # The root table (the last level's one table, 20,000 x 20,000): its
+    # device time per row is the latency one row of the column-blocked
+    # pipeline costs.
+    last = max(r["level"] for r in launches)
+    root = max((r for r in launches if r["level"] == last), key=lambda r: r["ms"])
+    with open(os.path.join(ROOT, "tests", "golden", "xl_expected.json")) as f:
+        want = json.load(f)["rrt20000_a500"]["dist"]
+    if dist != want:
+        raise SystemExit(f"20k pair: distance {dist}, the oracle's is {want}")
     return {"pair": "rrt20000_a500", "dist": dist, "ms": round(wall * 1e3, 2),
             "gcups": round(st.cells / wall / 1e9, 2),
             "hbm_gbs_algorithmic": round(BYTES_PER_CELL * st.cells / wall / 1e9, 1),
             "frac_of_hbm_peak": round(BYTES_PER_CELL * st.cells / wall / 1e9 / HBM_PEAK_GBS, 4),
             "cells": st.cells, "tables": st.tables, "levels": st.levels,
             "plan_ms": round(st.plan_ms, 2), "dp_ms": round(st.dp_ms, 2),
-            "column_blocked_ms": xl}
+            "column_blocked_ms": xl, "classes": classes,
+            "root_table": {"class": xted.class_name(root["cls"]), "ms": round(root["ms"], 3),
+                           "rows": 20000, "us_per_row": round(root["ms"] * 1e3 / 20000, 3)},
+            "checked": "distance against the oracle (tests/golden/xl_expected.json)"}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(trees, budget_s):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Worker threads for the CPU baseline: the host's cores, capped at the
+    CPU share a job gets beside one GPU (OMP_NUM_THREADS, 16 by default)."""
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(os.cpu_count() or 1, share))
+
+
+def cpu_baseline(trees, budget_s, threads):
     """Reference X-TED_CPU on the first pairs of the workload until the time
-    budget is spent (at least one pair)."""
+    budget is spent (at least one pair), one pair at a time as the
+    reference's driver runs them (X-TED_CPU/main.cpp:7-10). Timed end to end
+    (inputs -> distance and full table) and with the reference's own
+    compute-phase timer (X-TED_CPU/parallel_computing.cpp:156-170,184)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     if not oracle.ref_available():
         return None
-    threads = min(16, os.cpu_count() or 1)
-    done, cells, t0 = 0, 0, time.perf_counter()
+    done, cells, compute_ms, t0 = 0, 0, 0.0, time.perf_counter()
     while done < len(trees) // 2:
         x, y = trees[2 * done], trees[2 * done + 1]
-        oracle.ref_ted(x, y, threads=threads, want_td=True)
+        _, _, ms = oracle.ref_ted(x, y, threads=threads, want_td=True)
+        compute_ms += ms
         cells += xted.tree_stats(x[0])[0] * xted.tree_stats(y[0])[0]
         done += 1
         if time.perf_counter() - t0 > budget_s:
@@ -179,6 +256,9 @@ def cpu_baseline(trees, budget_s):
     dt = time.perf_counter() - t0
     return {"value": done / dt, "unit": "pairs/s", "cores": threads, "kind": "reference",
             "gcups": cells / dt / 1e9,
+            "compute_phase_pairs_per_s": done / (compute_ms / 1e3),
+            "compute_phase_gcups": cells / (compute_ms / 1e3) / 1e9,
+            "ms_per_pair": 1e3 * dt / done, "nproc": os.cpu_count(), "cpu_model": cpu_model(),
             "sample": f"first {done} pairs of the 4096-pair batch, one pair at a time "
                       f"(X-TED_CPU, {threads} threads, {dt:.1f} s)"}
 
@@ -230,6 +310,45 @@ def dominant_roofline(prof, suffix=""):
     return roofline, classes
 
 
+def level_profile(log1, log4):
+    """Per-level view of the event-timed steps (one launch record per forest
+    launch, in launch order). With one level stream each launch runs alone,
+    so a level's longest launch is the least time the level can take with
+    its launches overlapped: the sum over levels is the batch's critical
+    path under this plan (the latency bound), beside the time the levels
+    actually take on 4 streams. Levels whose launches hold fewer workgroups
+    than half the device's CUs (128 of 256) leave most CUs idle for their
+    whole span."""
+    def per_level(log):
+        lv = {}
+        for r in log:
+            d = lv.setdefault(r["level"], {"start": r["start_ms"], "end": 0.0, "longest": 0.0,
+                                           "tables": 0, "wgs": 0})
+            d["start"] = min(d["start"], r["start_ms"])
+            d["end"] = max(d["end"], r["start_ms"] + r["ms"])
+            d["longest"] = max(d["longest"], r["ms"])
+            d["tables"] += r["tables"]
+            d["wgs"] += -(-r["tables"] // 16) if r["cls"] == xted.CLASS_TINY else r["tables"]
+        return lv
+    l1, l4 = per_level(log1), per_level(log4)
+    span4 = {L: d["end"] - d["start"] for L, d in l4.items()}
+    total4 = max(d["end"] for d in l4.values()) - min(d["start"] for d in l4.values())
+    thin = [L for L, d in l4.items() if L >= 0 and d["wgs"] < 128]
+    return {
+        "levels": len([L for L in l4 if L >= 0]),
+        "critical_path_ms": round(sum(d["longest"] for L, d in l1.items()), 2),
+        "level_spans_ms_4_streams": round(sum(span4.values()), 2),
+        "dp_span_ms_4_streams": round(total4, 2),
+        "thin_levels": len(thin),
+        "thin_level_ms_4_streams": round(sum(span4[L] for L in thin), 2),
+        "thin_fraction": round(sum(span4[L] for L in thin) / total4, 4),
+        "per_level_ms_4_streams": [round(span4[L], 3) for L in sorted(span4) if L >= 0],
+        "per_level_longest_ms_1_stream": [round(l1[L]["longest"], 3) for L in sorted(l1)
+                                          if L >= 0],
+        "per_level_tables": [l4[L]["tables"] for L in sorted(l4) if L >= 0],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,6 +384,12 @@ def main():
                     help="skip the batch under a non-unit cost model")
     ap.add_argument("--serve-only", action="store_true",
                     help="only the serving leg (for profiling it alone)")
+    ap.add_argument("--serve-rate2", type=float, default=0.0,
+                    help="also serve a second stream at this offered rate (near saturation)")
+    ap.add_argument("--serve-seconds2", type=float, default=5.0)
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only event-timed steps on one level stream (the run to trace with "
+                         "rocprofv3 for the dominant kernel's per-dispatch duration)")
     args = ap.parse_args()
     if args.calibrate_copy:
         copy_calibration(args.calibrate_copy)
@@ -276,6 +401,10 @@ def main():
         strees = serving_trees(args.serve_trees)
         print(json.dumps({"serving": serve_bench(strees, args.serve_seconds, args.serve_rate,
                                                  budget=32 << 30, max_batch=4096)}), flush=True)
+        if args.serve_rate2 > 0:
+            print(json.dumps({"near_saturation": serve_bench(
+                strees, args.serve_seconds2, args.serve_rate2, budget=32 << 30, max_batch=4096,
+                seed=8)}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path "
@@ -298,7 +427,9 @@ def main():
         autotune.apply(ctx, rule)
     if args.batch_groups:
         ctx.set_batch_groups(args.batch_groups)
-    ctx.set_graphs(not args.no_graphs)
+    ctx.set_graphs(not args.no_graphs and not args.profile_only)
+    if args.profile_only:
+        ctx.set_level_streams(1)
     log(f"granularity rule: {rule}")
 
     dist = None
@@ -323,16 +454,44 @@ def main():
 
     if dist is None:
         dist, _, _ = ctx.batch(xs, ys)
+    # Every distance against the oracle's (committed fixture).
+    gold = golden_batch(args.pairs)
+    checked_batch = None
+    if gold is not None:
+        bad = np.flatnonzero(dist != gold)
+        if len(bad):
+            raise SystemExit(f"{len(bad)} batched distances differ from the oracle, "
+                             f"first pairs {bad[:8].tolist()}")
+        checked_batch = {"against": "oracle (tests/golden/batch4096_dist.npy)",
+                         "pairs": int(len(gold))}
+    if args.profile_only:
+        # Only event-timed steps on one level stream: the run rocprofv3
+        # traces for the dominant kernel's per-dispatch duration.
+        ctx.set_level_streams(1)
+        _, prof1, _ = timed(max(1, args.profile_steps), True, "profiled step (1 stream)", dist)
+        roof1, _ = dominant_roofline(prof1)
+        print(json.dumps({"roofline": roof1}), flush=True)
+        return
     # The timed steps: levels replayed from a captured graph unless
-    # --no-graphs. Then a second timed region with every forest launch
-    # bracketed by HIP events on its own stream (launched one by one): the
-    # per-class device times and the dominant kernel's launch durations.
+    # --no-graphs. Then timed regions with every forest launch bracketed by
+    # HIP events on its own stream (launched one by one): the per-class
+    # device times and the dominant kernel's launch durations, once with the
+    # launches of a level on one stream (each launch has the device to
+    # itself: the kernel's own rate) and once on the 4 level streams the
+    # timed steps use (launches of a level share the device).
     times, _, st = timed(args.steps, False, "step", dist)
+    ctx.set_level_streams(1)
+    ptimes1, prof1, _ = timed(max(1, args.profile_steps), True, "profiled step (1 stream)", dist)
+    log1 = ctx.launch_profile()
+    ctx.set_level_streams(4)
     ptimes, prof, _ = timed(max(1, args.profile_steps), True, "profiled step", dist)
+    log4 = ctx.launch_profile()
     ms_per_step = 1e3 * sum(times) / len(times)
     pairs_per_s = args.pairs / (ms_per_step / 1e3)
     gcups = st.cells / (ms_per_step / 1e3) / 1e9
-    roofline, classes = dominant_roofline(prof)
+    roofline, classes1 = dominant_roofline(prof1)
+    roofline4, classes = dominant_roofline(prof)
+    levels = level_profile(log1, log4)
 
     # The same batch under a non-unit cost model: the row-scan forms with
     # per-label cost lookups and prefix sums (the closed-form leaf tables and
@@ -388,7 +547,13 @@ def main():
     log(f"plain: {n_plain} pairs in {plain_s:.2f} s")
     ctx.close()
 
-    cpu = None if args.no_cpu else cpu_baseline(trees, args.cpu_budget)
+    cpu = None
+    if not args.no_cpu:
+        nthreads = cpu_threads()
+        cpu = cpu_baseline(trees, args.cpu_budget, nthreads)
+        if cpu is not None and (os.cpu_count() or 1) > nthreads:
+            cpu["note"] = (f"{nthreads} of the host's {os.cpu_count()} cores: the CPU share of a "
+                           f"job beside one GPU (OMP_NUM_THREADS)")
 
     xl = None
     if not args.no_xl:
@@ -403,6 +568,10 @@ def main():
         serving = serve_bench(strees, args.serve_seconds, args.serve_rate,
                               budget=32 << 30, max_batch=4096)
         log(f"serving: {serving}")
+        if args.serve_rate2 > 0:
+            serving["near_saturation"] = serve_bench(strees, args.serve_seconds2, args.serve_rate2,
+                                                     budget=32 << 30, max_batch=4096, seed=8)
+            log(f"serving near saturation: {serving['near_saturation']}")
 
     out = {
         "metric": "TED pairs/s (4096-pair uneven batch, 100-1000 nodes, batched persistent "
@@ -430,7 +599,14 @@ def main():
         "plain_sample_pairs": n_plain,
         "batched_vs_plain": round(pairs_per_s / plain_pairs_per_s, 2) if n_plain else None,
         "roofline": roofline,
+        "roofline_4_streams": roofline4,
         "classes": classes,
+        "classes_1_stream": classes1,
+        "profiled_ms_per_step_1_stream": round(1e3 * sum(ptimes1) / len(ptimes1), 3),
+        "level_profile": levels,
+        "checked": checked_batch,
+        "gcups_without_leaf_cells": round((st.cells - classes.get("leaf", {}).get("cells", 0))
+                                          / (ms_per_step / 1e3) / 1e9, 2),
         "cpu_baseline": cpu,
         "cost_model": cost_leg,
         "all_pairs": all_pairs,

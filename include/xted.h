@@ -131,8 +131,11 @@ int xted_batch(xted_context* ctx, int32_t npairs, const int32_t* x_ids, const in
  * trees, dist[i*ntrees + j] = TED(tree i, tree j) (trees concatenated as in
  * xted_pool_add). The reference's test() sizes a dimension x dimension
  * Distance matrix but fills only (0, 1) (X-TED_GPU/test_TED.cu:30-54). The
- * pairs run through the batched plan in chunks whose tree-distance tables
- * take at most max_bytes of HBM (0: 16 GiB). With unit costs the matrix is
+ * pairs run through the batched plan in chunks whose device bytes take at
+ * most max_bytes of HBM (0: 16 GiB): a pair counts its TD and TD^T (8*m*n
+ * bytes) and its plan arrays (72 bytes per planned keyroot-pair table: the
+ * tables of two non-leaf keyroots with unit costs, all K_x*K_y tables under
+ * a cost model). With unit costs the matrix is
  * symmetric with a zero diagonal and only i < j is computed; under a cost
  * model every ordered pair, the diagonal included, is. The context's pool
  * is used as scratch (cleared first). stats sums the chunks. */
@@ -209,11 +212,21 @@ int xted_set_costs(xted_context* ctx, int32_t alphabet, const int32_t* del, cons
 int xted_set_profile(xted_context* ctx, int32_t on);
 int xted_class_profile(xted_context* ctx, int32_t cls, int64_t* launches, int64_t* tables,
                        int64_t* cells, double* ms);
+/* Launch i of the last profiled call, in launch order: its level (-1 for the
+ * leaf kernel, which precedes level 0), class, tables, interior cells, start
+ * (ms after the call's first timed launch started) and device duration.
+ * *count receives the number of timed launches; i < 0 only sets it. With the
+ * level loop of X-TED_GPU/parallel_X_TED.cu:312-387 this gives each level's
+ * span and its longest launch (the batch's critical path). */
+int xted_launch_profile(xted_context* ctx, int32_t i, int32_t* level, int32_t* cls,
+                        int64_t* tables, int64_t* cells, double* start_ms, double* ms,
+                        int32_t* count);
 
 /* ---- continuous-batching server ---------------------------------------------------
  * No counterpart in the reference. A server thread drains submitted queries
  * over pool trees into device batches, admitting each by the HBM its TD and
- * TD^T tables need (8*m*n bytes) against hbm_budget bytes. */
+ * TD^T tables (8*m*n bytes) and its plan arrays (72 bytes per planned
+ * keyroot-pair table, as in xted_all_pairs) need against hbm_budget bytes. */
 typedef struct xted_server xted_server;
 typedef struct xted_result {
   uint64_t tag;         /* caller's tag from submit */

@@ -310,7 +310,8 @@ int xted_all_pairs(xted_context* ctx, int32_t ntrees, const int32_t* sizes,
     const bool sym = !c.cost_mode();
     const int64_t nn = (int64_t)ntrees * ntrees;
     std::fill(dist, dist + nn, 0);
-    // Pairs in row order, cut into chunks by their TD + TD^T bytes.
+    // Pairs in row order, cut into chunks by their device bytes (TD + TD^T
+    // and the plan arrays).
     std::vector<int32_t> xs, ys, out;
     int64_t bytes = 0;
     auto flush = [&]() -> int {
@@ -340,7 +341,7 @@ int xted_all_pairs(xted_context* ctx, int32_t ntrees, const int32_t* sizes,
     };
     for (int32_t i = 0; i < ntrees; ++i) {
       for (int32_t j = sym ? i + 1 : 0; j < ntrees; ++j) {
-        const int64_t b = 8 * (int64_t)sizes[i] * sizes[j] + 64;
+        const int64_t b = c.pair_device_bytes(ids[i], ids[j]);
         if (!xs.empty() && (bytes + b > max_bytes || xs.size() >= (size_t)(1 << 20))) {
           st = flush();
           if (st != OK) return st;
@@ -515,6 +516,30 @@ int xted_class_profile(xted_context* ctx, int32_t cls, int64_t* launches, int64_
   if (tables) *tables = cp.tables;
   if (cells) *cells = cp.cells;
   if (ms) *ms = cp.ms;
+  return OK;
+}
+
+int xted_launch_profile(xted_context* ctx, int32_t i, int32_t* level, int32_t* cls,
+                        int64_t* tables, int64_t* cells, double* start_ms, double* ms,
+                        int32_t* count) {
+  if (!ctx) {
+    set_error("xted_launch_profile: null context");
+    return ERR_INVALID_ARG;
+  }
+  const std::vector<LaunchRecord>& log = ctx->ctx.launch_log;
+  if (count) *count = (int32_t)log.size();
+  if (i < 0) return OK;
+  if (i >= (int32_t)log.size()) {
+    set_error("xted_launch_profile: launch index out of range");
+    return ERR_INVALID_ARG;
+  }
+  const LaunchRecord& r = log[i];
+  if (level) *level = r.level;
+  if (cls) *cls = r.cls;
+  if (tables) *tables = r.tables;
+  if (cells) *cells = r.cells;
+  if (start_ms) *start_ms = r.start_ms;
+  if (ms) *ms = r.ms;
   return OK;
 }
 

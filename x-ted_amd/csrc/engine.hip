@@ -782,6 +782,22 @@ int Context::sync() {
   return OK;
 }
 
+int Context::drain() {
+  HIP_TRY(hipSetDevice(device_));
+  HIP_TRY(hipStreamSynchronize(stream_));
+  for (hipStream_t sd : side_) HIP_TRY(hipStreamSynchronize(sd));
+  HIP_TRY(hipStreamSynchronize(plan_stream_));
+  return OK;
+}
+
+int64_t Context::pair_device_bytes(int32_t x, int32_t y) const {
+  const TreeMeta &a = meta_[x], &b = meta_[y];
+  const bool leaf_form = cost_alphabet_ == 0 && lane_enable;
+  const int64_t ka = leaf_form ? a.K - a.K1 : a.K, kb = leaf_form ? b.K - b.K1 : b.K;
+  return 8 * (int64_t)a.n * b.n + 3 * kTdPad * (int64_t)sizeof(int32_t) +
+         (int64_t)(sizeof(uint64_t) + sizeof(TaskDesc)) * ka * kb;
+}
+
 // Everything launch_levels bakes into the launches of a group: the plan's
 // counts and LDS sizes, the buffers, the pool arrays and the streams.
 std::vector<uint64_t> Context::graph_signature(const Group& g, int lane) const {
@@ -878,9 +894,13 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     plan_share = ng;
     st = plan(g, (int32_t)gx[k].size(), gx[k].data(), gy[k].data(), lane);
     plan_share = 1;
-    if (st != OK) return st;
-    st = run_levels(g, lane, profile ? &prof : nullptr);
-    if (st != OK) return st;
+    if (st == OK) st = run_levels(g, lane, profile ? &prof : nullptr);
+    if (st != OK) {
+      // Launches of this or an earlier group may still be in flight; the next
+      // call rewrites these groups' plans, so let them finish first.
+      (void)drain();
+      return st;
+    }
     HIP_TRY(hipEventRecord(g.ev_end, lane_stream(lane)));
   }
   // Distances, abort records and tables of every group, on its stream.
@@ -917,16 +937,22 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
 
   if (profile) {
     for (auto& cp : class_profile) cp = ClassProfile{};
+    launch_log.clear();
     for (size_t i = 0; i < prof.size(); ++i) {
-      float ms = 0;
+      float ms = 0, t0 = 0;
       HIP_TRY(hipEventElapsedTime(&ms, prof_ev_[2 * i], prof_ev_[2 * i + 1]));
+      HIP_TRY(hipEventElapsedTime(&t0, prof_ev_[0], prof_ev_[2 * i]));
       ClassProfile& cp = class_profile[prof[i].cls];
       const Group& g = *prof[i].g;
       cp.launches += 1;
       cp.ms += ms;
       const int b = prof[i].bucket;
-      cp.tables += b >= 0 ? (int64_t)g.h_counts[b] : g.leaf_tables;
-      cp.cells += b >= 0 ? (int64_t)g.h_cells[b] : g.leaf_cells;
+      const int64_t tables = b >= 0 ? (int64_t)g.h_counts[b] : g.leaf_tables;
+      const int64_t cells = b >= 0 ? (int64_t)g.h_cells[b] : g.leaf_cells;
+      cp.tables += tables;
+      cp.cells += cells;
+      launch_log.push_back(LaunchRecord{b >= 0 ? b / kNumClasses : -1, prof[i].cls, tables, cells,
+                                        (double)t0, (double)ms});
     }
   }
   // Plan time: the groups' plans added up; DP time: from the end of the

@@ -89,6 +89,15 @@ struct Group {
   void release();
 };
 
+// One timed launch of the last profiled call: level (-1 for the leaf
+// kernel), class, tables, interior cells, start (ms after the call's first
+// timed launch started) and device duration.
+struct LaunchRecord {
+  int32_t level, cls;
+  int64_t tables, cells;
+  double start_ms, ms;
+};
+
 // One timed forest launch: its class, its bucket (-1 for the leaf kernel)
 // and the group it belongs to.
 struct ProfLaunch {
@@ -126,6 +135,16 @@ class Context {
   // the alphabet.
   int64_t whole_tree_cost(const int32_t* labels, int32_t n, bool del) const;
   int32_t pool_size() const { return (int32_t)meta_.size(); }
+  // Device bytes a batch holds for pair (x, y) of the pool: its TD and TD^T
+  // (8*m*n plus padding) and its plan arrays, a task word and a 64-byte
+  // descriptor per planned keyroot-pair table (tables of two non-leaf
+  // keyroots with unit costs, every table under a cost model). Both ids
+  // must be valid; the caller holds mutex().
+  int64_t pair_device_bytes(int32_t x, int32_t y) const;
+  // Waits for everything launched on every stream of the context (the level
+  // streams and the plan stream), so that a group's buffers can be reused
+  // after a failed launch without in-flight kernels reading rewritten plans.
+  int drain();
   const TreeMeta& tree_meta(int32_t id) const { return meta_[id]; }
   const HostTreeInfo& tree_info(int32_t id) const { return info_[id]; }
   // Copies the device-computed orl, keyroots (descending) and keyroot depths.
@@ -185,6 +204,7 @@ class Context {
   // not while profiling, which times each launch with events.
   int32_t use_graphs = 1;
   ClassProfile class_profile[kNumClasses];
+  std::vector<LaunchRecord> launch_log;  // every timed launch of the last profiled call
 
   hipStream_t stream() const { return stream_; }
   int device() const { return device_; }

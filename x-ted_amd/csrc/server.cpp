@@ -20,13 +20,20 @@ Server::~Server() {
 }
 
 int Server::submit(int32_t x, int32_t y, uint64_t tag) {
-  if (x < 0 || y < 0 || x >= ctx_->pool_size() || y >= ctx_->pool_size()) {
-    set_error("server submit: tree id out of range");
-    return ERR_NOT_FOUND;
+  // The pool may grow concurrently (xted_pool_add): read it under the
+  // context's lock and keep copies.
+  int64_t bytes;
+  int lane;
+  {
+    std::lock_guard<std::mutex> lk(ctx_->mutex());
+    if (x < 0 || y < 0 || x >= ctx_->pool_size() || y >= ctx_->pool_size()) {
+      set_error("server submit: tree id out of range");
+      return ERR_NOT_FOUND;
+    }
+    const TreeMeta a = ctx_->tree_meta(x), b = ctx_->tree_meta(y);
+    bytes = ctx_->pair_device_bytes(x, y);
+    lane = (std::min(a.n, b.n) > kLightSide || a.kdmax + b.kdmax + 1 > kLightLevels);
   }
-  const TreeMeta &a = ctx_->tree_meta(x), &b = ctx_->tree_meta(y);
-  const int64_t bytes = 8 * (int64_t)a.n * b.n;
-  const int lane = (std::min(a.n, b.n) > kLightSide || a.kdmax + b.kdmax + 1 > kLightLevels);
   {
     std::lock_guard<std::mutex> lk(mu_);
     if (stop_) return ERR_CANCELLED;
@@ -139,7 +146,7 @@ void Server::admit(int lane) {
   }
   if (st != OK) {
     publish(batch, st, nullptr, t_admit);
-    spare_.push_back(std::move(g));
+    recycle(std::move(g));
     return;
   }
   inflight_bytes_ += bytes;
@@ -170,7 +177,7 @@ bool Server::tick(int lane) {
     }
     publish(a.reqs, ERR_INVALID_ARG, nullptr, a.t_admit);
     inflight_bytes_ -= a.bytes;
-    spare_.push_back(std::move(a.g));
+    recycle(std::move(a.g));
     L.active.erase(L.active.begin() + i);
   }
   if (L.active.empty()) return false;
@@ -196,13 +203,21 @@ bool Server::tick(int lane) {
     if (fs != OK) {
       publish(a.reqs, fs, nullptr, a.t_admit);
       inflight_bytes_ -= a.bytes;
-      spare_.push_back(std::move(a.g));
+      recycle(std::move(a.g));
     } else {
       completing_.push_back(std::move(a));
     }
     L.active.erase(L.active.begin() + i);
   }
   return true;
+}
+
+// A group that leaves on an error path may still have launches in flight
+// (earlier ticks, or launches of the failing tick made before the failure):
+// they finish before the group's buffers are handed to a new plan.
+void Server::recycle(std::unique_ptr<Group> g) {
+  (void)ctx_->drain();
+  spare_.push_back(std::move(g));
 }
 
 void Server::loop() {

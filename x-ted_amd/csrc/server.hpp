@@ -2,7 +2,8 @@
 //
 // Queries (x_id, y_id) are queued on submit. The server thread works in
 // ticks. Each tick it admits queued queries in arrival order while their
-// tables fit what is left of the HBM budget (TD + TD^T = 8*m*n bytes each)
+// device bytes fit what is left of the HBM budget (TD + TD^T, 8*m*n bytes,
+// plus the plan's 72 bytes per planned keyroot-pair table)
 // and plans them as a new group; then it launches the next level of every
 // group in flight together (levels of different groups are independent, so
 // their tables share the same launches' streams); a group whose last level
@@ -64,6 +65,7 @@ class Server {
   bool tick(int lane);  // false when nothing was launched
   void publish(const std::vector<Req>& reqs, int status, const int32_t* dist,
                Clock::time_point t_admit);
+  void recycle(std::unique_ptr<Group> g);  // error paths: drain, then keep the buffers
   Context* ctx_;
   int64_t budget_;
   int32_t max_batch_;

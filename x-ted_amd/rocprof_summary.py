@@ -38,28 +38,56 @@ def short(name):
     return base
 
 
+# Resource columns of a rocprofv3 kernel trace (names differ between
+# versions: the first one present is used).
+RESOURCES = [("vgpr", ("Arch_VGPR_Count", "VGPR_Count")),
+             ("agpr", ("Accum_VGPR_Count",)),
+             ("sgpr", ("SGPR_Count",)),
+             ("lds_B", ("LDS_Block_Size", "Group_Segment_Size")),
+             ("scratch_B", ("Scratch_Size", "Private_Segment_Size"))]
+
+
 def kernels(path, bench=None):
+    """Per kernel: dispatches, average and total duration, the registers,
+    LDS and scratch of its dispatches (largest over them: LDS is dynamic),
+    and with the bench line the algorithmic bytes per launch. The bench
+    line's class cells come from its event-timed steps: `classes_1_stream`
+    when present (the run traced with one level stream), else `classes`,
+    plus the 20,000-node pair's column-blocked classes (`xl_pair.classes`)."""
     per = defaultdict(list)
+    res = defaultdict(dict)
     with open(path) as f:
         for r in csv.DictReader(f):
-            per[short(r["Kernel_Name"])].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+            k = short(r["Kernel_Name"])
+            per[k].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+            for name, cols in RESOURCES:
+                for col in cols:
+                    if r.get(col) not in (None, ""):
+                        res[k][name] = max(res[k].get(name, 0), int(float(r[col])))
+                        break
     classes = {}
     if bench:
         with open(bench) as f:
             for line in f:
                 if line.startswith("{"):
                     j = json.loads(line)
-                    classes = dict(j.get("classes", {}))
+                    if "roofline" in j and "classes" not in j and "classes_1_stream" not in j:
+                        continue
+                    classes = dict(j.get("classes_1_stream") or j.get("classes", {}))
                     cm = j.get("cost_model") or {}
                     classes.update({k + "_cost": v for k, v in cm.get("classes", {}).items()})
+                    classes.update((j.get("xl_pair") or {}).get("classes", {}))
     rows = sorted(per.items(), key=lambda kv: -sum(kv[1]))
-    out = [f"{'kernel':<26}{'dispatches':>11}{'avg_us':>11}{'total_ms':>11}"
-           f"{'alg_MB/launch':>15}{'alg_GB/s':>10}"]
+    head = f"{'kernel':<26}{'dispatches':>11}{'avg_us':>11}{'total_ms':>11}"
+    rcols = [name for name, _ in RESOURCES if any(name in v for v in res.values())]
+    head += "".join(f"{c:>10}" for c in rcols)
+    out = [head + f"{'alg_MB/launch':>15}{'alg_GB/s':>10}"]
     for k, d in rows:
         avg = sum(d) / len(d) / 1e3
         line = f"{k:<26}{len(d):>11}{avg:>11.1f}{sum(d) / 1e6:>11.2f}"
+        line += "".join(f"{res[k].get(c, ''):>10}" for c in rcols)
         c = classes.get(k)
-        if c and k.startswith("rowscan"):
+        if c and (k.startswith("rowscan") or k.startswith("xl_bw")):
             b = 4 * c["cells"] / c["launches"]
             line += f"{b / 1e6:>15.1f}{b / (avg * 1e3):>10.1f}"
         out.append(line)

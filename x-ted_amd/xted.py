@@ -94,6 +94,11 @@ _SIGNATURES = {
                                           ctypes.POINTER(ctypes.c_int64),
                                           ctypes.POINTER(ctypes.c_int64),
                                           ctypes.POINTER(ctypes.c_double)]),
+    "xted_launch_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32P, _I32P,
+                                           ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double), _I32P]),
     "xted_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_void_p)]),
     "xted_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
@@ -297,7 +302,8 @@ class Context:
         return n.value
 
     def batch(self, xs, ys, td_shapes=None):
-        """Distances of pool pairs; td_shapes (list of (m, n)) also returns the tables."""
+        """Distances of pool pairs; td_shapes (list of (m, n), one per pair,
+        None for a pair whose table is not wanted) also returns the tables."""
         xs = np.ascontiguousarray(xs, dtype=np.int32)
         ys = np.ascontiguousarray(ys, dtype=np.int32)
         dist = np.empty(len(xs), np.int32)
@@ -305,8 +311,11 @@ class Context:
         tds = None
         arr = None
         if td_shapes is not None:
-            tds = [np.empty(s, np.int32) for s in td_shapes]
-            arr = (ctypes.c_void_p * len(tds))(*[t.ctypes.data for t in tds])
+            if len(td_shapes) != len(xs):
+                raise ValueError("td_shapes: one entry per pair")
+            tds = [None if s is None else np.empty(s, np.int32) for s in td_shapes]
+            arr = (ctypes.c_void_p * len(tds))(*[None if t is None else t.ctypes.data
+                                                 for t in tds])
         _check(lib().xted_batch(self._h, len(xs), _p(xs), _p(ys), _p(dist),
                                 ctypes.cast(arr, ctypes.c_void_p) if arr is not None else None,
                                 ctypes.byref(st)))
@@ -370,6 +379,25 @@ class Context:
                                             ctypes.byref(ce), ctypes.byref(ms)))
             if la.value:
                 out[c] = (la.value, ta.value, ce.value, ms.value)
+        return out
+
+    def launch_profile(self):
+        """Every timed launch of the last profiled call, in launch order:
+        dicts of level (-1: the leaf kernel), cls, tables, cells, start_ms
+        (after the first timed launch started) and ms (device time)."""
+        count = ctypes.c_int32()
+        _check(lib().xted_launch_profile(self._h, -1, None, None, None, None, None, None,
+                                         ctypes.byref(count)))
+        out = []
+        for i in range(count.value):
+            lv, cl = ctypes.c_int32(), ctypes.c_int32()
+            ta, ce = ctypes.c_int64(), ctypes.c_int64()
+            t0, ms = ctypes.c_double(), ctypes.c_double()
+            _check(lib().xted_launch_profile(self._h, i, ctypes.byref(lv), ctypes.byref(cl),
+                                             ctypes.byref(ta), ctypes.byref(ce), ctypes.byref(t0),
+                                             ctypes.byref(ms), None))
+            out.append({"level": lv.value, "cls": cl.value, "tables": ta.value,
+                        "cells": ce.value, "start_ms": t0.value, "ms": ms.value})
         return out
 
     def force_class(self, cls=CLASS_AUTO, lane_enable=True):
