@@ -176,6 +176,14 @@ int xted_set_level_streams(xted_context* ctx, int32_t n);
  * plan overlaps the levels of the groups before it. No counterpart in the
  * reference, which runs one pair at a time. Results do not depend on it. */
 int xted_set_batch_groups(xted_context* ctx, int32_t groups, int32_t split_min);
+/* Where each pair's levels sit among the batch's levels: 0 (the default)
+ * starts every pair at level 0, as the reference's single pair does
+ * (X-TED_GPU/parallel_X_TED.cu:214-220); 1 ends every pair at the batch's
+ * last level; 2 places it midway. A pair's own levels keep their order
+ * (level = depth_x + depth_y + offset), so results do not depend on it; the
+ * offsets move the tables of shallow pairs into the levels whose time is set
+ * by the few large tables of the deepest pairs. */
+int xted_set_level_align(xted_context* ctx, int32_t mode);
 /* on = 1 (the default): the level launches of a batch without
  * column-blocked tables are captured once as a HIP graph and re-launched
  * while a later batch plans to the same launches (same counts per level and

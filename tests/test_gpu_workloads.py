@@ -64,6 +64,23 @@ def test_batch4096_tuned_rule_every_distance(batch4096):
         assert len(bad) == 0, f"{len(bad)} distances differ, first pairs {bad[:8].tolist()}"
 
 
+@pytest.mark.parametrize("align", [1, 2])
+def test_batch4096_level_alignment(batch4096, align):
+    """Pairs placed to end at the batch's last level (1) or midway (2)
+    instead of starting at level 0: every distance against the oracle's."""
+    ctx, trees, xs, ys = batch4096
+    expect = np.load(os.path.join(GOLDEN, "batch4096_dist.npy"))
+    ctx.set_level_align(align)
+    try:
+        print(f"batch4096: level alignment {align}", flush=True)
+        d, _, st = ctx.batch(xs, ys)
+    finally:
+        ctx.set_level_align(autotune.load()["level_align"])
+    assert st.levels == 116
+    bad = np.flatnonzero(d != expect)
+    assert len(bad) == 0, f"{len(bad)} distances differ, first pairs {bad[:8].tolist()}"
+
+
 def test_batch4096_sample_against_live_oracle(batch4096_sample):
     ctx, trees, xs, ys, sample, d = batch4096_sample
     for k in sample:

@@ -25,7 +25,7 @@ sys.path.insert(0, HERE)
 import xted  # noqa: E402
 
 DEFAULT = {"lds_per_wave": 10 * 1024, "max_cpl": 8, "min_waves": 1, "wide_cols": 1 << 30,
-           "tail_pairs": 32}
+           "tail_pairs": 32, "level_align": 0}
 # Searched in stages: the tail rule first, then the throughput rule with the
 # best tail setting.
 STAGES = [
@@ -38,6 +38,7 @@ TUNING_FILE = os.path.join(HERE, "tuning.json")
 def apply(ctx, rule):
     ctx.set_tuning(rule["lds_per_wave"], rule["max_cpl"], rule["min_waves"], rule["wide_cols"])
     ctx.set_tail_rule(rule["tail_pairs"])
+    ctx.set_level_align(rule["level_align"])
 
 
 def load(path=TUNING_FILE):

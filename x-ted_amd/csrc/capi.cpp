@@ -483,6 +483,15 @@ int xted_set_batch_groups(xted_context* ctx, int32_t groups, int32_t split_min) 
   return OK;
 }
 
+int xted_set_level_align(xted_context* ctx, int32_t mode) {
+  if (!ctx || mode < 0 || mode > 2) {
+    set_error("xted_set_level_align: mode must be 0, 1 or 2");
+    return ERR_INVALID_ARG;
+  }
+  ctx->ctx.level_align = mode;
+  return OK;
+}
+
 int xted_set_costs(xted_context* ctx, int32_t alphabet, const int32_t* del, const int32_t* ins,
                    const int32_t* ren) {
   return guarded([&]() -> int {

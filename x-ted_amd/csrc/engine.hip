@@ -390,7 +390,8 @@ void Group::release() {
   graph = nullptr;
   graph_sig.clear();
   for (DevBuf* b : {&pairs, &blk, &counts, &cells, &cursor, &maxlds, &tickets, &abort, &tasks,
-                    &descs, &td, &dist, &lines_d, &tail_lv, &xl_off, &xl_total, &xl_words})
+                    &descs, &td, &dist, &lines_d, &tail_lv, &xl_off, &xl_total, &xl_words,
+                    &loff_d})
     b->release();
 }
 
@@ -480,9 +481,17 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   }
   g.td_bytes = (int64_t)sizeof(int32_t) * td_total;
   const int nb = nlevels * kNumClasses;
+  g.loff.assign(npairs, 0);
+  if (level_align == 1 || level_align == 2) {
+    for (int32_t i = 0; i < npairs; ++i) {
+      const int32_t spare = nlevels - (meta_[xs[i]].kdmax + meta_[ys[i]].kdmax + 1);
+      g.loff[i] = level_align == 1 ? spare : spare / 2;
+    }
+  }
   // Levels that at most tail_pairs pairs reach use the latency rule.
   std::vector<int32_t> active(nlevels + 1, 0);
-  for (int32_t i = 0; i < npairs; ++i) ++active[meta_[xs[i]].kdmax + meta_[ys[i]].kdmax];
+  for (int32_t i = 0; i < npairs; ++i)
+    ++active[meta_[xs[i]].kdmax + meta_[ys[i]].kdmax + g.loff[i]];
   g.h_tail.assign(nlevels, 0);
   for (int L = nlevels - 1, run = 0; L >= 0; --L) {
     run += active[L];
@@ -502,6 +511,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   ST_TRY(g.td.grow((size_t)g.td_bytes, false, ps));
   ST_TRY(g.dist.grow(sizeof(int32_t) * npairs, false, ps));
   ST_TRY(g.lines_d.grow(sizeof(int64_t) * (npairs + 1), false, ps));
+  if (level_align) ST_TRY(g.loff_d.grow(sizeof(int32_t) * npairs, false, ps));
   HIP_TRY(hipMemcpyAsync(g.pairs.p, g.pm.data(), sizeof(PairMeta) * npairs,
                          hipMemcpyHostToDevice, ps));
   HIP_TRY(hipMemcpyAsync(g.blk.p, blk.data(), sizeof(int64_t) * (npairs + 1),
@@ -509,6 +519,9 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   HIP_TRY(hipMemcpyAsync(g.lines_d.p, g.lines.data(), sizeof(int64_t) * (npairs + 1),
                          hipMemcpyHostToDevice, ps));
   HIP_TRY(hipMemcpyAsync(g.tail_lv.p, g.h_tail.data(), nlevels, hipMemcpyHostToDevice, ps));
+  if (level_align)
+    HIP_TRY(hipMemcpyAsync(g.loff_d.p, g.loff.data(), sizeof(int32_t) * npairs,
+                           hipMemcpyHostToDevice, ps));
   for (hipEvent_t* e : {&g.ev_plan0, &g.ev_plan1, &g.ev_end})
     if (!*e) HIP_TRY(hipEventCreate(e));
   HIP_TRY(hipEventRecord(g.ev_plan0, ps));
@@ -535,6 +548,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   pa.pool = view();
   pa.pairs = g.pairs.as<PairMeta>();
   pa.blk_start = g.blk.as<int64_t>();
+  pa.level_off = level_align ? g.loff_d.as<int32_t>() : nullptr;
   pa.npairs = npairs;
   pa.nlevels = nlevels;
   pa.lchunk_tables = plan_chunk;

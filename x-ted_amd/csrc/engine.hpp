@@ -64,11 +64,12 @@ struct Group {
   std::vector<int32_t> xs, ys;
   std::vector<PairMeta> pm;
   std::vector<int64_t> lines, xl_first;  // lines: prefix of leaf-kernel tiles per pair
+  std::vector<int32_t> loff;             // per pair: level offset (Context::level_align)
   std::vector<unsigned long long> h_counts, h_cells;
   std::vector<int32_t> h_maxlds;
   std::vector<uint8_t> h_tail;
   DevBuf pairs, blk, counts, cells, cursor, maxlds, tickets, tasks, descs, td, dist, lines_d,
-      tail_lv, xl_off, xl_total;
+      tail_lv, xl_off, xl_total, loff_d;
   DevBuf abort, xl_words;
   // finish_async: distances then the abort flag land in pinned memory
   int32_t* h_dist_pin = nullptr;
@@ -203,6 +204,12 @@ class Context {
   // captured graph (re-launched while the plan's launches stay the same);
   // not while profiling, which times each launch with events.
   int32_t use_graphs = 1;
+  // Where a pair's levels sit among the batch's: 0 = every pair starts at
+  // level 0, 1 = every pair ends at the batch's last level, 2 = midway. A
+  // pair's own levels keep their order, so each is a valid schedule; the
+  // choice moves shallow pairs' tables into levels whose time is set by the
+  // few large tables of the deepest pairs.
+  int32_t level_align = 0;
   ClassProfile class_profile[kNumClasses];
   std::vector<LaunchRecord> launch_log;  // every timed launch of the last profiled call
 

@@ -53,6 +53,9 @@ struct PlanArgs {
   PoolView pool;
   const PairMeta* pairs;
   const int64_t* blk_start;    // npairs+1: first block of each pair
+  // Per pair: added to the level of each of its tables (null: none). The
+  // levels of one pair keep their order, so any offset is a valid schedule.
+  const int32_t* level_off;
   int32_t npairs;
   int32_t nlevels;
   int32_t lchunk_tables;       // target tables per block

@@ -30,14 +30,14 @@ __device__ __forceinline__ int find_pair(const int64_t* blk_start, int npairs, i
 
 // Per-table classification shared by both passes.
 __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
-                                         const TreeMeta& ty, int k, int l, int* bucket,
+                                         const TreeMeta& ty, int32_t loff, int k, int l, int* bucket,
                                          int* lds, uint32_t* i0, uint32_t* j0,
                                          unsigned long long* cells) {
   const int64_t kx = tx.node_off + k, ly = ty.node_off + l;
   const int32_t a = A.pool.ssize[kx], b = A.pool.ssize[ly];
   *cells = (unsigned long long)a * (unsigned long long)b;
   const int32_t kdx = A.pool.skd[kx], kdy = A.pool.skd[ly];
-  const int32_t level = kdx + kdy;
+  const int32_t level = kdx + kdy + loff;
   const ClassRule& rule = A.tail_level[level] ? A.rule_tail : A.rule;
   // Slot storage of the oriented table: one uint16 row per nesting level of
   // the row keyroot (at least one: the kernels read slot 0 unconditionally),
@@ -277,6 +277,7 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(PlanArgs A) {
   const PairMeta pm = A.pairs[p];
   const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
   const PlanChunk ch = plan_chunk(A, tx, ty, p);
+  const int32_t loff = A.level_off ? A.level_off[p] : 0;
   ht_clear(T);
   __syncthreads();
   auto sweep = [&](auto body) {
@@ -288,7 +289,7 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(PlanArgs A) {
       unsigned long long cells = 0;
       if (active) {
         const int l = ch.lbase + ch.l_lo + (int)(idx / ch.Kx), k = ch.kbase + (int)(idx % ch.Kx);
-        classify(A, tx, ty, k, l, &bucket, &lds, &i0, &j0, &cells);
+        classify(A, tx, ty, loff, k, l, &bucket, &lds, &i0, &j0, &cells);
       }
       body(active, bucket, lds, i0, j0, cells);
     }

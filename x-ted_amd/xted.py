@@ -78,6 +78,7 @@ _SIGNATURES = {
     "xted_set_level_streams": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_batch_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "xted_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "xted_set_level_align": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_bracket_to_lines": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64,
                                              ctypes.c_char_p, ctypes.c_int64,
                                              ctypes.POINTER(ctypes.c_int64),
@@ -353,6 +354,11 @@ class Context:
 
     def set_graphs(self, on=True):
         _check(lib().xted_set_graphs(self._h, 1 if on else 0))
+
+    def set_level_align(self, mode):
+        """0: every pair starts at level 0; 1: every pair ends at the last
+        level; 2: midway."""
+        _check(lib().xted_set_level_align(self._h, mode))
 
     def set_costs(self, costs=None):
         """costs = (dele, ins, ren) per-label int32 arrays (alphabet, alphabet,
