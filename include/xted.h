@@ -127,6 +127,21 @@ int xted_pool_size(xted_context* ctx, int32_t* ntrees);
  * m x n table. */
 int xted_batch(xted_context* ctx, int32_t npairs, const int32_t* x_ids, const int32_t* y_ids,
                int32_t* dist, int32_t* const* td_out, xted_stats* stats);
+/* xted_batch without device allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Every
+ * device buffer the batch needs (plan arrays, TD arena, distances) is carved
+ * from one fixed array of XTED_NOALLOC_BYTES (64 MiB) that the library holds
+ * on each device it runs on until the process ends; the call makes no
+ * hipMalloc / hipFree / pinned-host allocation and instantiates no graph
+ * (its levels are launched one by one). Pairs whose buffers exceed the array
+ * (8*m*n bytes of tables plus 72 bytes per planned keyroot-pair table, as in
+ * xted_all_pairs) are refused with XTED_ERR_TOO_LARGE. Distances only. Calls
+ * on one context are serialised; calls on different contexts of one device
+ * share that device's array and must not overlap. */
+#define XTED_NOALLOC_BYTES (64 << 20)
+int xted_batch_noalloc(xted_context* ctx, int32_t npairs, const int32_t* x_ids,
+                       const int32_t* y_ids, int32_t* dist);
 /* All-pairs / dataset join: the ntrees x ntrees distance matrix of a set of
  * trees, dist[i*ntrees + j] = TED(tree i, tree j) (trees concatenated as in
  * xted_pool_add). The reference's test() sizes a dimension x dimension

@@ -288,6 +288,15 @@ int xted_batch(xted_context* ctx, int32_t npairs, const int32_t* x_ids, const in
   });
 }
 
+int xted_batch_noalloc(xted_context* ctx, int32_t npairs, const int32_t* x_ids,
+                       const int32_t* y_ids, int32_t* dist) {
+  return guarded([&]() -> int {
+    if (!ctx) return ERR_INVALID_ARG;
+    std::lock_guard<std::recursive_mutex> api(ctx->api);
+    return ctx->ctx.run_batch_noalloc(npairs, x_ids, y_ids, dist);
+  });
+}
+
 int xted_all_pairs(xted_context* ctx, int32_t ntrees, const int32_t* sizes,
                    const int32_t* parents, const int32_t* labels, int64_t max_bytes,
                    int32_t* dist, xted_stats* stats) {

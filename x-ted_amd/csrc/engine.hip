@@ -44,6 +44,19 @@ constexpr int32_t kMaxTreeNodes = 32767;
 
 int DevBuf::grow(size_t bytes, bool keep, hipStream_t s) {
   if (bytes <= cap) return OK;
+  if (arena) {
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (arena->used + need > arena->cap) {
+      set_error("no-allocation batch: the pairs need more than the " +
+                std::to_string(arena->cap) + " bytes of the device scratch array (" +
+                std::to_string(arena->used + need) + " bytes so far)");
+      return ERR_TOO_LARGE;
+    }
+    p = arena->base + arena->used;
+    arena->used += need;
+    cap = need;
+    return OK;
+  }
   size_t ncap = std::max(bytes, cap + cap / 2);
   ncap = (ncap + 4095) & ~(size_t)4095;
   void* np = nullptr;
@@ -66,10 +79,15 @@ int DevBuf::grow(size_t bytes, bool keep, hipStream_t s) {
 }
 
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p && !arena) (void)hipFree(p);
   p = nullptr;
   cap = 0;
 }
+
+// The no-allocation batch path's buffers, one array per device (each device
+// holds its own copy). Host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (Context::init).
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 Context::Context(int device) : device_(device) {}
 
@@ -79,6 +97,7 @@ Context::~Context() {
     (void)hipDeviceSynchronize();
   }
   for (Group& g : batch_) g.release();
+  na_.release();
   for (DevBuf* b : {&parent_, &label_, &orl_, &kdep_, &kdn_, &lrank_, &kr_, &krkd_, &krsize_,
                     &krleaf_, &trees_, &prep_scratch_, &prep_off_, &skr_, &skd_, &ssize_, &sleaf_,
                     &spre_, &kdoff_, &rtop_, &cdel_, &cins_, &cren_, &pdel_, &pins_, &ctot_})
@@ -123,6 +142,13 @@ int Context::init() {
   lds_limit = (int32_t)std::min<size_t>(kLdsMaxTable, prop.sharedMemPerBlock - 256);
   HIP_TRY(forest_init_attributes(lds_limit));
   rule.lds_max = lds_limit;
+  // This device's copy of the no-allocation array, and what its batch group
+  // needs besides memory, so that run_batch_noalloc creates nothing.
+  void* sp = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  na_arena_ = Arena{static_cast<char*>(sp), kNoAllocBytes, 0};
+  na_.for_each_buf([this](DevBuf& b) { b.arena = &na_arena_; });
+  for (hipEvent_t* e : {&na_.ev_plan0, &na_.ev_plan1, &na_.ev_end}) HIP_TRY(hipEventCreate(e));
   return OK;
 }
 
@@ -389,10 +415,7 @@ void Group::release() {
   graph_exec = nullptr;
   graph = nullptr;
   graph_sig.clear();
-  for (DevBuf* b : {&pairs, &blk, &counts, &cells, &cursor, &maxlds, &tickets, &abort, &tasks,
-                    &descs, &td, &dist, &lines_d, &tail_lv, &xl_off, &xl_total, &xl_words,
-                    &loff_d})
-    b->release();
+  for_each_buf([](DevBuf& b) { b.release(); });
 }
 
 int Context::lane_streams(int lane, hipStream_t* out) const {
@@ -989,6 +1012,83 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   out.total_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   if (stats) *stats = out;
+  return OK;
+}
+
+// Bytes plan() carves for these pairs, every buffer rounded to 256 bytes as
+// the arena does (the column-blocked words, sized on the device, come on top).
+int64_t Context::noalloc_bytes(int32_t npairs, const int32_t* xs, const int32_t* ys) const {
+  auto r = [](int64_t b) { return (b + 255) & ~int64_t{255}; };
+  const bool le = cost_alphabet_ == 0 && lane_enable;
+  int64_t td = 0, planned = 0;
+  int32_t nlevels = 0;
+  for (int32_t i = 0; i < npairs; ++i) {
+    const TreeMeta &a = meta_[xs[i]], &b = meta_[ys[i]];
+    td += 2 * (int64_t)a.n * b.n + 3 * kTdPad;
+    planned += (int64_t)(le ? a.K - a.K1 : a.K) * (le ? b.K - b.K1 : b.K);
+    nlevels = std::max(nlevels, a.kdmax + b.kdmax + 1);
+  }
+  const int64_t nb = (int64_t)nlevels * kNumClasses;
+  planned = std::max<int64_t>(planned, 1);
+  return r(sizeof(PairMeta) * npairs) + r(nlevels) + 2 * r(8 * (int64_t)(npairs + 1)) +
+         3 * r(8 * nb) + 2 * r(4 * nb) + r(32) + r(8 * planned) + r(64 * planned) + r(4 * td) +
+         r(4 * (int64_t)npairs) + (level_align ? r(4 * (int64_t)npairs) : 0);
+}
+
+int Context::run_batch_noalloc(int32_t npairs, const int32_t* xs, const int32_t* ys,
+                               int32_t* dist) {
+  std::lock_guard<std::mutex> lk(mu_);
+  if (npairs < 0 || (npairs > 0 && (!xs || !ys || !dist))) {
+    set_error("batch_noalloc: null argument");
+    return ERR_INVALID_ARG;
+  }
+  if (npairs == 0) return OK;
+  if (!na_arena_.base) {
+    set_error("batch_noalloc: the context has no device scratch array");
+    return ERR_DEVICE;
+  }
+  for (int32_t i = 0; i < npairs; ++i) {
+    if (xs[i] < 0 || ys[i] < 0 || xs[i] >= (int32_t)meta_.size() ||
+        ys[i] >= (int32_t)meta_.size()) {
+      set_error("batch_noalloc: tree id out of range in pair " + std::to_string(i));
+      return ERR_NOT_FOUND;
+    }
+  }
+  const int64_t need = noalloc_bytes(npairs, xs, ys);
+  if (need > (int64_t)na_arena_.cap) {
+    set_error("batch_noalloc: the pairs need " + std::to_string(need) +
+              " bytes of device scratch; the array holds " + std::to_string(na_arena_.cap));
+    return ERR_TOO_LARGE;
+  }
+  HIP_TRY(hipSetDevice(device_));
+  // Every buffer is carved afresh from the start of the array.
+  na_arena_.used = 0;
+  na_.for_each_buf([](DevBuf& b) {
+    b.p = nullptr;
+    b.cap = 0;
+  });
+  // No graph (instantiating one allocates) and no per-launch events.
+  const int32_t graphs = use_graphs, prof = profile;
+  use_graphs = 0;
+  profile = 0;
+  int st = plan(na_, npairs, xs, ys, -1);
+  if (st == OK) st = run_levels(na_, -1, nullptr);
+  use_graphs = graphs;
+  profile = prof;
+  if (st != OK) {
+    (void)drain();  // launches made before the failure finish first
+    return st;
+  }
+  hipStream_t s = lane_stream(-1);
+  int32_t rec[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  launch_gather_dist(na_.pairs.as<PairMeta>(), na_.td.as<int32_t>(), npairs,
+                     na_.dist.as<int32_t>(), s);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(dist, na_.dist.p, sizeof(int32_t) * npairs, hipMemcpyDeviceToHost, s));
+  if (na_.nxl > 0)
+    HIP_TRY(hipMemcpyAsync(rec, na_.abort.p, sizeof(rec), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (rec[0]) return xl_timeout(rec);
   return OK;
 }
 

@@ -22,6 +22,10 @@ const char* last_error();
 // process by default.
 constexpr int kMaxLevelStreams = 4;
 
+// Device bytes of the fixed array the no-allocation batch path carves its
+// buffers from (one per device, held until the process ends).
+constexpr size_t kNoAllocBytes = size_t{64} << 20;
+
 struct BatchStats {
   double plan_ms = 0;      // device plan build incl. the count read-back
   double dp_ms = 0;        // level loop (all forest kernels)
@@ -32,11 +36,22 @@ struct BatchStats {
   int32_t launches = 0;    // forest kernel launches
 };
 
-// Growable device buffer. grow() keeps contents when `keep` is set.
+// A fixed device region that buffers are carved from in order (no
+// allocation): the no-allocation batch path takes every buffer from it.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+};
+
+// Growable device buffer. grow() keeps contents when `keep` is set. A buffer
+// bound to an arena never allocates: grow() carves the next piece of the
+// arena (contents are not kept) or fails with ERR_TOO_LARGE, and release()
+// only forgets the piece.
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
   bool fine = false;  // fine-grained allocation (hipDeviceMallocFinegrained)
+  Arena* arena = nullptr;
   int grow(size_t bytes, bool keep, hipStream_t s);
   void release();
   template <class T>
@@ -88,6 +103,14 @@ struct Group {
   Group& operator=(const Group&) = delete;
   ~Group() { release(); }
   void release();
+  // Every device buffer of the group.
+  template <class F>
+  void for_each_buf(F f) {
+    for (DevBuf* b : {&pairs, &blk, &counts, &cells, &cursor, &maxlds, &tickets, &abort, &tasks,
+                      &descs, &td, &dist, &lines_d, &tail_lv, &xl_off, &xl_total, &xl_words,
+                      &loff_d})
+      f(*b);
+  }
 };
 
 // One timed launch of the last profiled call: level (-1 for the leaf
@@ -154,6 +177,14 @@ class Context {
   // Evaluation -------------------------------------------------------------
   // Distances of pairs (xs[i], ys[i]); when td_out is given, td_out[i] (may be
   // null per pair) receives the m x n tree-distance table.
+  // run_batch without any device allocation: every buffer of the batch (plan,
+  // TD arena, distances) is carved from the fixed device array of
+  // kNoAllocBytes this library holds per device (its address is looked up
+  // once, at init); launches go one by one (no graph is instantiated), the
+  // distances are copied to the caller's memory directly. Pairs that do not
+  // fit are refused with ERR_TOO_LARGE. Calls that overlap on one device
+  // (from different contexts) share the array and race.
+  int run_batch_noalloc(int32_t npairs, const int32_t* xs, const int32_t* ys, int32_t* dist);
   int run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int32_t* dist,
                 int32_t* const* td_out, BatchStats* stats);
 
@@ -233,6 +264,9 @@ class Context {
   DevBuf skr_, skd_, ssize_, sleaf_, spre_, kdoff_, rtop_;
   DevBuf prep_scratch_, prep_off_;
   Group batch_[kMaxLevelStreams];  // the groups run_batch reuses
+  Group na_;                        // run_batch_noalloc's group, bound to na_arena_
+  Arena na_arena_;                  // this device's fixed array (kNoAllocBytes)
+  int64_t noalloc_bytes(int32_t npairs, const int32_t* xs, const int32_t* ys) const;
   uint32_t xl_epoch_ = 0;  // tag of the latest column-blocked launch
   std::vector<hipEvent_t> prof_ev_;
   int grow_pool(int64_t nodes);

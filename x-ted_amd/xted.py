@@ -71,6 +71,7 @@ _SIGNATURES = {
     "xted_pool_size": (ctypes.c_int, [ctypes.c_void_p, _I32P]),
     "xted_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32P, _I32P, _I32P,
                                   ctypes.c_void_p, ctypes.c_void_p]),
+    "xted_batch_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32P, _I32P, _I32P]),
     "xted_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.c_int32, ctypes.c_int32]),
     "xted_force_class": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
@@ -321,6 +322,15 @@ class Context:
                                 ctypes.cast(arr, ctypes.c_void_p) if arr is not None else None,
                                 ctypes.byref(st)))
         return dist, tds, st
+
+    def batch_noalloc(self, xs, ys):
+        """Distances of pool pairs through xted_batch_noalloc: no device
+        allocation, every buffer from the library's fixed per-device array."""
+        xs = np.ascontiguousarray(xs, dtype=np.int32)
+        ys = np.ascontiguousarray(ys, dtype=np.int32)
+        dist = np.empty(len(xs), np.int32)
+        _check(lib().xted_batch_noalloc(self._h, len(xs), _p(xs), _p(ys), _p(dist)))
+        return dist
 
     def all_pairs(self, trees, max_bytes=0):
         """The len(trees) x len(trees) distance matrix of trees ((parent,
