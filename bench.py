@@ -387,6 +387,9 @@ def main():
     ap.add_argument("--serve-rate2", type=float, default=0.0,
                     help="also serve a second stream at this offered rate (near saturation)")
     ap.add_argument("--serve-seconds2", type=float, default=5.0)
+    ap.add_argument("--dump-launches", default="",
+                    help="write the per-launch records of the event-timed steps (1 and 4 level "
+                         "streams) to this JSON file")
     ap.add_argument("--profile-only", action="store_true",
                     help="only event-timed steps on one level stream (the run to trace with "
                          "rocprofv3 for the dominant kernel's per-dispatch duration)")
@@ -492,6 +495,9 @@ def main():
     roofline, classes1 = dominant_roofline(prof1)
     roofline4, classes = dominant_roofline(prof)
     levels = level_profile(log1, log4)
+    if args.dump_launches:
+        with open(args.dump_launches, "w") as f:
+            json.dump({"one_stream": log1, "four_streams": log4}, f)
 
     # The same batch under a non-unit cost model: the row-scan forms with
     # per-label cost lookups and prefix sums (the closed-form leaf tables and

@@ -199,6 +199,12 @@ int xted_set_batch_groups(xted_context* ctx, int32_t groups, int32_t split_min);
  * offsets move the tables of shallow pairs into the levels whose time is set
  * by the few large tables of the deepest pairs. */
 int xted_set_level_align(xted_context* ctx, int32_t mode);
+/* on = 1 (the default): the tables of each launch start in four size bands,
+ * longest first (by rows), so a launch ends on short tables; 0: in plan
+ * order. The reference sorts each level's queue by table size, ascending
+ * (thrust::sort_by_key, X-TED_GPU/parallel_X_TED.cu:374). Results do not
+ * depend on it. */
+int xted_set_size_bands(xted_context* ctx, int32_t on);
 /* on = 1 (the default): the level launches of a batch without
  * column-blocked tables are captured once as a HIP graph and re-launched
  * while a later batch plans to the same launches (same counts per level and

@@ -81,6 +81,21 @@ def test_batch4096_level_alignment(batch4096, align):
     assert len(bad) == 0, f"{len(bad)} distances differ, first pairs {bad[:8].tolist()}"
 
 
+def test_batch4096_plan_order(batch4096):
+    """Tables of each launch in plan order instead of largest first: every
+    distance against the oracle's."""
+    ctx, trees, xs, ys = batch4096
+    expect = np.load(os.path.join(GOLDEN, "batch4096_dist.npy"))
+    ctx.set_size_bands(False)
+    try:
+        print("batch4096: plan order", flush=True)
+        d, _, _ = ctx.batch(xs, ys)
+    finally:
+        ctx.set_size_bands(autotune.load()["size_bands"])
+    bad = np.flatnonzero(d != expect)
+    assert len(bad) == 0, f"{len(bad)} distances differ, first pairs {bad[:8].tolist()}"
+
+
 def test_batch4096_sample_against_live_oracle(batch4096_sample):
     ctx, trees, xs, ys, sample, d = batch4096_sample
     for k in sample:

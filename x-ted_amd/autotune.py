@@ -25,7 +25,7 @@ sys.path.insert(0, HERE)
 import xted  # noqa: E402
 
 DEFAULT = {"lds_per_wave": 10 * 1024, "max_cpl": 8, "min_waves": 1, "wide_cols": 1 << 30,
-           "tail_pairs": 32, "level_align": 0}
+           "tail_pairs": 32, "level_align": 0, "size_bands": 1}
 # Searched in stages: the tail rule first, then the throughput rule with the
 # best tail setting.
 STAGES = [
@@ -39,6 +39,7 @@ def apply(ctx, rule):
     ctx.set_tuning(rule["lds_per_wave"], rule["max_cpl"], rule["min_waves"], rule["wide_cols"])
     ctx.set_tail_rule(rule["tail_pairs"])
     ctx.set_level_align(rule["level_align"])
+    ctx.set_size_bands(rule["size_bands"])
 
 
 def load(path=TUNING_FILE):

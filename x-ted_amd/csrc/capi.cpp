@@ -492,6 +492,12 @@ int xted_set_batch_groups(xted_context* ctx, int32_t groups, int32_t split_min) 
   return OK;
 }
 
+int xted_set_size_bands(xted_context* ctx, int32_t on) {
+  if (!ctx) return ERR_INVALID_ARG;
+  ctx->ctx.size_bands = on ? 1 : 0;
+  return OK;
+}
+
 int xted_set_level_align(xted_context* ctx, int32_t mode) {
   if (!ctx || mode < 0 || mode > 2) {
     set_error("xted_set_level_align: mode must be 0, 1 or 2");

@@ -127,6 +127,21 @@ constexpr int kFirstSeqXLForm = 2;
 constexpr int kMaxImports = 256;
 XTED_HD inline int row_class(int wi, int ci) { return 1 + wi * 5 + ci; }
 
+// Each (level, class) bucket is cut into kBands size bands, largest tables
+// first, so that a launch starts its longest tables first and its last
+// workgroups are short ones (column-blocked classes use band 0 only). The
+// plan's buckets are (level, class, band), and one launch covers the bands
+// of a (level, class).
+constexpr int kBands = 4;
+XTED_HD inline int size_band(int cls, int rows) {
+  if (cls == kClassTiny) return rows >= 13 ? 0 : rows >= 9 ? 1 : rows >= 5 ? 2 : 3;
+  if (cls < 1 || cls > kNumRowClasses) return 0;
+  return rows >= 256 ? 0 : rows >= 64 ? 1 : rows >= 16 ? 2 : 3;
+}
+XTED_HD inline int bucket_of(int level, int cls, int band) {
+  return (level * kNumClasses + cls) * kBands + band;
+}
+
 // Saved-row slots are uint16 (F <= |subtree x| + |subtree y| < 65536).
 // LDS budget for the slots of one table, per wave of its workgroup.
 constexpr int kLdsPerWave = 10 * 1024;
@@ -156,10 +171,10 @@ XTED_HD inline int ilog2_ceil(int v) {
 
 // Slot bytes of one block of a column-blocked table: one uint16 row per
 // nesting level of the row keyroot, each holding the block's own far columns
-// (at most min(leaves, bw)), the boundary entry and kd_c+1 imports.
+// (at most min(leaves, bw)), the boundary entry, kd_c+1 imports and a sink.
 XTED_HD inline int64_t xl_lds_need(int kd_r, int kd_c, int leaves_c, int bw) {
   const int own = leaves_c < bw ? leaves_c : bw;
-  return (int64_t)(kd_r > 1 ? kd_r : 1) * (own + 2 + kd_c) * 2;
+  return (int64_t)(kd_r > 1 ? kd_r : 1) * (own + 3 + kd_c) * 2;
 }
 
 // Class of table (a x b): lds_need is the slot storage of its oriented

@@ -503,7 +503,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
     st.cells += info_[x].ksum * info_[y].ksum;
   }
   g.td_bytes = (int64_t)sizeof(int32_t) * td_total;
-  const int nb = nlevels * kNumClasses;
+  const int nb = nlevels * kNumClasses * kBands;
   g.loff.assign(npairs, 0);
   if (level_align == 1 || level_align == 2) {
     for (int32_t i = 0; i < npairs; ++i) {
@@ -579,6 +579,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   pa.rule_tail = ClassRule{lds_limit, 1, 1 << 30, lds_limit, 16, rule.xl_seq_rows};
   pa.tail_level = g.tail_lv.as<uint8_t>();
   pa.force_class = force_class;
+  pa.bands = size_bands;
   pa.lane_enable = le;
   pa.counts = g.counts.as<unsigned long long>();
   pa.cells = g.cells.as<unsigned long long>();
@@ -607,14 +608,14 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   g.xl_first.assign(nb, 0);
   g.nxl = 0;
   for (int L = 0; L < nlevels; ++L) {
-    if (g.h_counts[(size_t)L * kNumClasses + kClassXL]) {
+    if (g.h_counts[bucket_of(L, kClassXL, 0)]) {
       set_error("run_batch: a table fits no form (nesting height above " +
                 std::to_string(kMaxImports - 1) + " or slot storage above " +
                 std::to_string(lds_limit) + " bytes even in 256-column blocks)");
       return ERR_TOO_LARGE;
     }
     for (int k = 0; k < kNumXLForms; ++k) {
-      const size_t bkt = (size_t)L * kNumClasses + kFirstXLClass + k;
+      const size_t bkt = bucket_of(L, kFirstXLClass + k, 0);
       g.xl_first[bkt] = g.nxl;
       g.nxl += (int64_t)g.h_counts[bkt];
     }
@@ -663,8 +664,10 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
   // of few large tables overlap with the rest instead of queueing behind it.
   struct Launch {
     Group* g;
-    int cls, bucket;
+    int cls, bucket;  // bucket: the (level, class) bucket's band 0
     unsigned long long off, cnt, cells;
+    int32_t lds;      // largest slot storage over the bands
+    int level;
   };
   std::vector<Launch> lv, leaf;
   for (int gi = 0; gi < ng; ++gi) {
@@ -681,12 +684,20 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
     // tables with placeholders, so it runs before the group's first level.
     if (L == 0 && g.lines[g.npairs] > 0 && !g.leaf_early)
       leaf.push_back({&g, kClassLane, -1, 0, (unsigned long long)g.lines[g.npairs],
-                      (unsigned long long)g.leaf_cells});
+                      (unsigned long long)g.leaf_cells, 0, -1});
     unsigned long long off = g.level_off;
     for (int c = 0; c < kNumClasses; ++c) {
-      const size_t b = (size_t)L * kNumClasses + c;
-      if (g.h_counts[b]) lv.push_back({&g, c, (int)b, off, g.h_counts[b], g.h_cells[b]});
-      off += g.h_counts[b];
+      // The bands of (L, c) are contiguous in the task array, largest first.
+      unsigned long long cnt = 0, cells = 0;
+      int32_t lds = 0;
+      for (int bd = 0; bd < kBands; ++bd) {
+        const size_t b = bucket_of(L, c, bd);
+        cnt += g.h_counts[b];
+        cells += g.h_cells[b];
+        lds = std::max(lds, g.h_maxlds[b]);
+      }
+      if (cnt) lv.push_back({&g, c, bucket_of(L, c, 0), off, cnt, cells, lds, L});
+      off += cnt;
     }
     g.level_off = off;
     ++g.next_level;
@@ -750,7 +761,7 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
         da.xl_epoch = xl_epoch_;
         da.xl_nbmax = (g.maxn + bw - 1) / bw;
       }
-      const int lds = std::max(64, (g.h_maxlds[w.bucket] + 15) & ~15);
+      const int lds = std::max(64, (w.lds + 15) & ~15);
       if (!launch_forest(w.cls, da, lds, s)) {
         set_error("run_batch: no kernel for class " + std::to_string(w.cls));
         return ERR_INVALID_ARG;
@@ -758,7 +769,9 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
     }
     if (prof) {
       HIP_TRY(hipEventRecord(prof_ev_[2 * prof->size() + 1], s));
-      prof->push_back(ProfLaunch{w.cls, w.bucket, &g});
+      prof->push_back(ProfLaunch{w.cls, w.level,
+                                 w.cls == kClassLane ? g.leaf_tables : (int64_t)w.cnt,
+                                 (int64_t)w.cells, &g});
     }
     ++g.st.launches;
   }
@@ -980,15 +993,11 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
       HIP_TRY(hipEventElapsedTime(&ms, prof_ev_[2 * i], prof_ev_[2 * i + 1]));
       HIP_TRY(hipEventElapsedTime(&t0, prof_ev_[0], prof_ev_[2 * i]));
       ClassProfile& cp = class_profile[prof[i].cls];
-      const Group& g = *prof[i].g;
       cp.launches += 1;
       cp.ms += ms;
-      const int b = prof[i].bucket;
-      const int64_t tables = b >= 0 ? (int64_t)g.h_counts[b] : g.leaf_tables;
-      const int64_t cells = b >= 0 ? (int64_t)g.h_cells[b] : g.leaf_cells;
-      cp.tables += tables;
-      cp.cells += cells;
-      launch_log.push_back(LaunchRecord{b >= 0 ? b / kNumClasses : -1, prof[i].cls, tables, cells,
+      cp.tables += prof[i].tables;
+      cp.cells += prof[i].cells;
+      launch_log.push_back(LaunchRecord{prof[i].level, prof[i].cls, prof[i].tables, prof[i].cells,
                                         (double)t0, (double)ms});
     }
   }
@@ -1028,7 +1037,7 @@ int64_t Context::noalloc_bytes(int32_t npairs, const int32_t* xs, const int32_t*
     planned += (int64_t)(le ? a.K - a.K1 : a.K) * (le ? b.K - b.K1 : b.K);
     nlevels = std::max(nlevels, a.kdmax + b.kdmax + 1);
   }
-  const int64_t nb = (int64_t)nlevels * kNumClasses;
+  const int64_t nb = (int64_t)nlevels * kNumClasses * kBands;
   planned = std::max<int64_t>(planned, 1);
   return r(sizeof(PairMeta) * npairs) + r(nlevels) + 2 * r(8 * (int64_t)(npairs + 1)) +
          3 * r(8 * nb) + 2 * r(4 * nb) + r(32) + r(8 * planned) + r(64 * planned) + r(4 * td) +

@@ -122,10 +122,11 @@ struct LaunchRecord {
   double start_ms, ms;
 };
 
-// One timed forest launch: its class, its bucket (-1 for the leaf kernel)
-// and the group it belongs to.
+// One timed forest launch: its class, level (-1 for the leaf kernel), tables,
+// interior cells and the group it belongs to.
 struct ProfLaunch {
-  int cls, bucket;
+  int cls, level;
+  int64_t tables, cells;
   const Group* g;
 };
 
@@ -241,6 +242,9 @@ class Context {
   // choice moves shallow pairs' tables into levels whose time is set by the
   // few large tables of the deepest pairs.
   int32_t level_align = 0;
+  // 1: the tables of a launch start largest first (size bands, kBands);
+  // 0: in plan order.
+  int32_t size_bands = 1;
   ClassProfile class_profile[kNumClasses];
   std::vector<LaunchRecord> launch_log;  // every timed launch of the last profiled call
 

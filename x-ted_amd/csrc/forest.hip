@@ -352,7 +352,6 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // row r-1's total while a slower wave still reads row r's.
   __shared__ int32_t xtot[2][W];
   __shared__ int32_t xsh[kXL ? kMaxImports + 1 : 1];  // import flags, then the ticket
-  __shared__ uint16_t sink[64];  // target of the saved-row stores a lane does not make
 
   int64_t tix = blockIdx.x;
   int32_t blk = 0, ticket = 0;
@@ -438,7 +437,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     lbase = dsc.lbase;
     wb = lrankC[cmax] - lbase - 1;
   }
-  const int32_t rowlen = wb + 1 + kds;
+  // The row's last entry is a sink: the saved-row stores of columns that
+  // save nothing go there, so every column stores without a branch.
+  const int32_t sinkix = wb + 1 + kds;
+  const int32_t rowlen = sinkix + 1;
 
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -446,22 +448,37 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
 #define TCOL(g, e) (tlane + (g) * 64 * E + (e))
 
   // Per-column state, element j = g*E + e (t = TCOL(g, e), c = cmax-1-t).
-  //   bfar: F[rmax][orl_C(c)+1] = cmax - orl_C(c) - 1 (the far value on the
-  //         boundary row), or -1 on a tree column (orl_C(c) == orl_C(c0));
-  //   labc: the column label (tree cells only);
+  //   tcol: a tree column (orl_C(c) == orl_C(c0));
+  //   lb:   on a tree column its label (tree cells only), else the far value
+  //         on the boundary row F[rmax][orl_C(c)+1] = cmax - orl_C(c) - 1;
   //   foff: the column's far index in a slot row (wb for tree columns);
   //   woff: the index this column writes in a saved row (c-1 a leaf of
-  //         subtree(c0)), or -1;
+  //         subtree(c0)), or the sink entry;
   //   xinf: (kXL) for a written column whose right path's top keyroot lies
   //         in a later block: that block << 16 | the path's depth index;
   //         else -1;
-  //   Fpm:  F[r+1][c] + 1 - t (the scan works on a[t] - t);
-  //   negt: -t.
+  //   Fpm:  F[r+1][c] + 1 - t + T_g (the scan works on a[t] - t);
+  //   negt: (kCost) the column's -Q(c).
+  // Unit costs keep the values of group g shifted by T_g = TCOL(g, 0), this
+  // lane's t at the group's first element: the per-column term -t + T_g is
+  // then the constant -e (an inline operand, not a register), and only the
+  // wave scan of each group works unshifted. term(j) is that per-column term
+  // (-Q(c) under a cost model, where nothing is shifted), gsh(g) the shift.
   // Columns past the table (t >= ncols) come after every real column in the
   // scan order, so whatever they hold never reaches a real column; only
   // their stores are masked.
-  int32_t bfar[C], labc[C], foff[C], woff[C], Fpm[C], negt[C], tdq[S][C];
+  bool tcol[C];
+  int32_t lb[C], foff[C], woff[C], Fpm[C], tdq[S][C];
+  int32_t negt[kCost ? C : 1];
   int32_t xinf[kXL ? C : 1];
+  auto term = [&](int j) -> int32_t {
+    if constexpr (kCost) return negt[j];
+    else return -(j % E);
+  };
+  auto gsh = [&](int g) -> int32_t {
+    if constexpr (kCost) return 0;
+    else return TCOL(g, 0);
+  };
   // F[rmax][c] + (vertical cost of row rmax-1) - Q(c): 2 with unit costs.
   const int32_t fpm0 = kCost ? (hc - hq) + (vr - Vpre[rmax - 1]) : 2;
 #pragma unroll
@@ -477,16 +494,19 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       const int32_t ocm = orlC[max(c - 1, c0)];
       const int32_t lrm = lrankC[max(c - 1, c0)];
       const bool tree = (oc == orl_c0);
-      bfar[j] = tree ? -1 : kCost ? hc - Hpre[oc + 1] : cmax - oc - 1;
-      labc[j] = lab;
+      tcol[j] = tree;
+      lb[j] = tree ? lab : kCost ? hc - Hpre[oc + 1] : cmax - oc - 1;
       foff[j] = tree ? wb : lro - lbase;
       const bool writes = t < ncols && c > c0 && ocm == c - 1;
-      woff[j] = writes ? lrm - lbase : -1;
-      Fpm[j] = fpm0;  // F[rmax][c] + 1 - t = cmax - c + 1 - t with unit costs
-      negt[j] = kCost ? Hpre[c] - hq : -t;
-      // Opaque to the optimiser, so that sums with -t stay three-operand
-      // adds (v_add3_u32) instead of becoming an add and a subtract.
-      asm("" : "+v"(negt[j]));
+      woff[j] = writes ? lrm - lbase : sinkix;
+      // F[rmax][c] + 1 - t = cmax - c + 1 - t with unit costs, shifted
+      Fpm[j] = fpm0 + gsh(g);
+      if constexpr (kCost) {
+        negt[j] = Hpre[c] - hq;
+        // Opaque to the optimiser, so that sums with it stay three-operand
+        // adds (v_add3_u32) instead of becoming an add and a subtract.
+        asm("" : "+v"(negt[j]));
+      }
       if constexpr (kXL) {
         // far column oc+1 in an earlier block: an import, by path depth
         const bool imp = !tree && t < ncols && cmax - 2 - oc < tb0;
@@ -726,8 +746,8 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           for (int g = 0; g < G; ++g) {
             const int jl = g * E + E - 1;  // the group's last column
             const int32_t fill =
-                g ? __builtin_amdgcn_readlane(Fpm[jl - E] - vcur - negt[jl - E], 63) : left_prev;
-            fpr0[g] = wave_shr1(Fpm[jl] - vcur - negt[jl], fill);
+                g ? __builtin_amdgcn_readlane(Fpm[jl - E] - vcur - term(jl - E), 63) : left_prev;
+            fpr0[g] = wave_shr1(Fpm[jl] - vcur - term(jl), fill);
           }
         }
 #pragma unroll
@@ -742,26 +762,27 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
               // Blended with a mask rather than a branch per element.
               int32_t fprm, rename;
               if constexpr (kCost) {
-                fprm = e ? Fpm[j - 1] - vcur - negt[j - 1] + negt[j] : fpr0[g] + negt[j];
+                fprm = e ? Fpm[j - 1] - vcur - term(j - 1) + term(j) : fpr0[g] + term(j);
                 // Columns off the tree path read entry 0 (one line for the
                 // wave) instead of a row of the matrix.
-                rename = A.ren[(rows_x ? labr * A.alphabet + labc[j] : labc[j] * A.alphabet + labr) &
-                               (bfar[j] >> 31)];
+                rename = A.ren[tcol[j] ? (rows_x ? labr * A.alphabet + lb[j]
+                                                 : lb[j] * A.alphabet + labr)
+                                       : 0];
               } else {
-                fprm = e ? Fpm[j - 1] - 2 : fpr0[g] + negt[j];
-                rename = (int32_t)(labr != labc[j]);
+                fprm = e ? Fpm[j - 1] - 2 : fpr0[g] + term(j);
+                rename = (int32_t)(labr != lb[j]);
               }
               const int32_t diag = fprm + rename;
-              const int32_t far = bfar[j] + tdq[u][j] + negt[j];
-              const int32_t m = bfar[j] >> 31;  // all ones on a tree column
-              dm = (diag & m) | (far & ~m);
+              const int32_t far = lb[j] + tdq[u][j] + term(j);
+              // Blended rather than branched per element.
+              dm = tcol[j] ? diag : far;
             } else {
-              dm = sv[j] + tdq[u][j] + negt[j];
+              dm = sv[j] + tdq[u][j] + term(j);
             }
             h[j] = min(dm, Fpm[j]);
             if (e) h[j] = min(h[j], h[j - 1]);
           }
-          const int32_t incl = wave_incl_min(h[g * E + E - 1]);
+          const int32_t incl = wave_incl_min(h[g * E + E - 1] - gsh(g));
           // (values stay far below kBig, so group 0 needs no min with it)
           pre[g] = g ? min(carry_w, wave_shr1(incl, kBig)) : wave_shr1(incl, kBig);
           carry_w = g ? min(carry_w, __builtin_amdgcn_readlane(incl, 63))
@@ -803,7 +824,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       // h becomes G - t; Fpm = G + 1 - t = h + 1.
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const int32_t cg = min(carry, pre[g]);
+        const int32_t cg = min(carry, pre[g]) + gsh(g);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           const int j = g * E + e;
@@ -814,24 +835,28 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
 
       // Tree cells -> TD / TD^T (tree rows only); saved rows -> LDS slots.
       if (tree_row) {
+        // Column of element (0, 0), made opaque here so that the compiler
+        // forms these rare stores' addresses when they happen instead of
+        // keeping two 64-bit addresses per element live across the loop.
+        int32_t cb = cmax - 1 - tlane;
+        asm volatile("" : "+v"(cb));
+        int32_t* const rowp = TDo + (int64_t)r * ldo;
+        int32_t* const colp = TDt + r;
+        const uint32_t ldt32 = (uint32_t)ldt;
 #pragma unroll
         for (int j = 0; j < C; ++j) {
-          const int32_t t = TCOL(j / E, j % E);
-          if (t < ncols && bfar[j] < 0) {
-            const int32_t c = cmax - 1 - t;
-            TDo[(int64_t)r * ldo + c] = h[j] - negt[j];
-            TDt[(int64_t)c * ldt + r] = h[j] - negt[j];
+          const int32_t c = cb - ((j / E) * 64 * E + j % E);
+          if (c >= c0 && tcol[j]) {
+            // c * ldt < 2^30: trees have at most 32767 nodes
+            rowp[c] = h[j] - term(j);
+            colp[(uint32_t)c * ldt32] = h[j] - term(j);
           }
         }
       }
       if (wrow) {
-        // Columns this lane does not save write to a sink entry of their
-        // own instead of branching per element.
+        // Columns this lane does not save write the row's sink entry.
 #pragma unroll
-        for (int j = 0; j < C; ++j) {
-          uint16_t* dst = woff[j] >= 0 ? wrow_ptr + woff[j] : sink + lane;
-          *dst = (uint16_t)(h[j] - negt[j]);
-        }
+        for (int j = 0; j < C; ++j) wrow_ptr[woff[j]] = (uint16_t)(h[j] - term(j));
         if (threadIdx.x == 0) wrow_ptr[wb] = (uint16_t)frc;
         if constexpr (kXL) {
           // Imported far values of this saved row; far values later
@@ -855,7 +880,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
             if (xinf[j] >= 0) {
               const int32_t kx = xinf[j] & 0xFFFF;
               for (int32_t bb = blk + 1; bb <= (xinf[j] >> 16); ++bb)
-                xl_store(xword(rid, bb, kx), A.xl_epoch, h[j] - negt[j], seq);
+                xl_store(xword(rid, bb, kx), A.xl_epoch, h[j] - term(j), seq);
             }
           }
         }
@@ -1068,7 +1093,7 @@ __global__ __launch_bounds__(1024) void xl_layout_kernel(XLLayoutArgs A) {
   int64_t ord = 0;
   for (int L = 0; L < A.nlevels; ++L) {
     for (int k = 0; k < kNumXLForms; ++k) {
-      const int bkt = L * kNumClasses + kFirstXLClass + k;
+      const int bkt = bucket_of(L, kFirstXLClass + k, 0);
       const int64_t cnt = (int64_t)A.counts[bkt];
       const int64_t start = (int64_t)A.cursor[bkt] - cnt;  // cursors end past the bucket
       for (int64_t c0 = 0; c0 < cnt; c0 += 1024) {

@@ -65,14 +65,15 @@ struct PlanArgs {
   ClassRule rule_tail;
   const uint8_t* tail_level;   // nlevels
   int32_t force_class;         // -1 = automatic; else row-scan tables go to this class if they fit
+  int32_t bands;               // 0: every table in band 0 (plan order within a launch)
   int32_t lane_enable;         // 0 = single-row tables use the row-scan kernels
-  unsigned long long* counts;  // nlevels*kNumClasses
-  unsigned long long* cells;   // nlevels*kNumClasses: interior cells per bucket
+  unsigned long long* counts;  // per bucket (bucket_of: level, class, band)
+  unsigned long long* cells;   // per bucket: interior cells
   unsigned long long* cursor;  // scatter cursors (bucket offsets)
-  int32_t* maxlds;             // nlevels*kNumClasses: largest slot storage per bucket
+  int32_t* maxlds;             // per bucket: largest slot storage
   uint64_t* tasks;
   TaskDesc* descs;             // one per task, same order
-  unsigned int* tickets;       // nlevels*kNumClasses: column-blocked launch counters
+  unsigned int* tickets;       // per bucket: column-blocked launch counters
   int32_t* abort_flag;         // 8 ints: column-blocked waits gave up, and what they waited for
 };
 // Blocks a pair occupies in the plan kernels (same formula on both sides).

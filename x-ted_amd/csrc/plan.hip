@@ -41,10 +41,10 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
   const ClassRule& rule = A.tail_level[level] ? A.rule_tail : A.rule;
   // Slot storage of the oriented table: one uint16 row per nesting level of
   // the row keyroot (at least one: the kernels read slot 0 unconditionally),
-  // each as long as the column keyroot's leaf count.
+  // each as long as the column keyroot's leaf count plus one sink entry.
   const int32_t slots = max(a <= b ? kdx : kdy, 1);
   const int32_t leaves_c = a <= b ? A.pool.sleaf[ly] : A.pool.sleaf[kx];
-  int64_t need = (int64_t)slots * leaves_c * 2;
+  int64_t need = (int64_t)slots * (leaves_c + 1) * 2;
   // With the leaf kernel off, single-row tables take the row-scan forms.
   int cls = table_class(a, b, need, a <= b ? kdx : kdy, a <= b ? kdy : kdx, leaves_c,
                         A.lane_enable != 0, rule, &need);
@@ -63,7 +63,7 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
   // can hold.
   if (A.force_class == kClassTiny && cls >= 1 && (cls <= kNumRowClasses || cls == kClassTiny)) {
     const int cols = a <= b ? b : a;
-    const int64_t row_need = (int64_t)slots * leaves_c * 2;
+    const int64_t row_need = (int64_t)slots * (leaves_c + 1) * 2;
     if (cols <= kTinyCols && row_need <= kTinyMaxLds) {
       cls = kClassTiny;
       need = row_need;
@@ -73,14 +73,14 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
       (cls <= kNumRowClasses || cls == kClassTiny)) {
     const int wi = (A.force_class - 1) / 5, ci = (A.force_class - 1) % 5;
     const int cols = a <= b ? b : a;
-    const int64_t row_need = (int64_t)slots * leaves_c * 2;
+    const int64_t row_need = (int64_t)slots * (leaves_c + 1) * 2;
     if ((cls < kClassXL || cls == kClassTiny) && cols <= 64 * (1 << wi) * (1 << ci) &&
         row_need <= A.rule.lds_max) {
       cls = A.force_class;
       need = row_need;
     }
   }
-  *bucket = level * kNumClasses + cls;
+  *bucket = bucket_of(level, cls, A.bands ? size_band(cls, a <= b ? a : b) : 0);
   *lds = (int)need;
   *i0 = (uint32_t)A.pool.skr[kx];
   *j0 = (uint32_t)A.pool.skr[ly];
@@ -367,7 +367,7 @@ __global__ void plan_clear_kernel(PlanArgs a, int nb) {
 }
 
 void launch_plan(const PlanArgs& a, int64_t nblocks, hipStream_t s) {
-  const int nb = a.nlevels * kNumClasses;
+  const int nb = a.nlevels * kNumClasses * kBands;
   hipLaunchKernelGGL(plan_clear_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, a, nb);
   if (nblocks > 0)
     hipLaunchKernelGGL(plan_kernel<false>, dim3((unsigned)nblocks), dim3(kPlanThreads), 0, s, a);

@@ -80,6 +80,7 @@ _SIGNATURES = {
     "xted_set_batch_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "xted_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_level_align": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "xted_set_size_bands": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_bracket_to_lines": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64,
                                              ctypes.c_char_p, ctypes.c_int64,
                                              ctypes.POINTER(ctypes.c_int64),
@@ -364,6 +365,10 @@ class Context:
 
     def set_graphs(self, on=True):
         _check(lib().xted_set_graphs(self._h, 1 if on else 0))
+
+    def set_size_bands(self, on=True):
+        """Tables of a launch start longest first (four size bands)."""
+        _check(lib().xted_set_size_bands(self._h, 1 if on else 0))
 
     def set_level_align(self, mode):
         """0: every pair starts at level 0; 1: every pair ends at the last
