@@ -58,6 +58,10 @@ typedef struct xted_stats {
   int64_t tables;   /* keyroot-pair tables */
   int32_t levels;
   int32_t launches; /* forest kernel launches */
+  /* column-blocked tables: cross-workgroup reads issued ahead that were
+   * stale and had to be polled again, and pipeline checks that waited */
+  int64_t xl_stale_reads;
+  int64_t xl_waits;
 } xted_stats;
 
 /* ---- context --------------------------------------------------------------

@@ -51,6 +51,8 @@ void to_c(const BatchStats& s, xted_stats* out) {
   out->tables = s.tables;
   out->levels = s.levels;
   out->launches = s.launches;
+  out->xl_stale_reads = s.xl_stale;
+  out->xl_waits = s.xl_waits;
 }
 
 }  // namespace
@@ -342,6 +344,8 @@ int xted_all_pairs(xted_context* ctx, int32_t ntrees, const int32_t* sizes,
         stats->tables += bs.tables;
         stats->levels = std::max<int32_t>(stats->levels, bs.levels);
         stats->launches += bs.launches;
+        stats->xl_stale_reads += bs.xl_stale;
+        stats->xl_waits += bs.xl_waits;
       }
       xs.clear();
       ys.clear();

@@ -528,7 +528,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   ST_TRY(g.cursor.grow(sizeof(unsigned long long) * nb, false, ps));
   ST_TRY(g.maxlds.grow(sizeof(int32_t) * nb, false, ps));
   ST_TRY(g.tickets.grow(sizeof(unsigned int) * nb, false, ps));
-  ST_TRY(g.abort.grow(sizeof(int32_t) * 8, false, ps));
+  ST_TRY(g.abort.grow(sizeof(int32_t) * kAbortInts, false, ps));
   ST_TRY(g.tasks.grow(sizeof(uint64_t) * std::max<int64_t>(planned, 1), false, ps));
   ST_TRY(g.descs.grow(sizeof(TaskDesc) * std::max<int64_t>(planned, 1), false, ps));
   ST_TRY(g.td.grow((size_t)g.td_bytes, false, ps));
@@ -790,7 +790,7 @@ int Context::finish_async(Group& g, int lane) {
     g.h_dist_pin = nullptr;
     g.pin_cap = 0;
     HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.h_dist_pin),
-                          sizeof(int32_t) * (g.npairs + 8), hipHostMallocDefault));
+                          sizeof(int32_t) * (g.npairs + kAbortInts), hipHostMallocDefault));
     g.pin_cap = g.npairs;
   }
   if (!g.done_ev) HIP_TRY(hipEventCreateWithFlags(&g.done_ev, hipEventDisableTiming));
@@ -799,7 +799,7 @@ int Context::finish_async(Group& g, int lane) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(g.h_dist_pin, g.dist.p, sizeof(int32_t) * g.npairs,
                          hipMemcpyDeviceToHost, ls));
-  HIP_TRY(hipMemcpyAsync(g.h_dist_pin + g.npairs, g.abort.p, sizeof(int32_t) * 8,
+  HIP_TRY(hipMemcpyAsync(g.h_dist_pin + g.npairs, g.abort.p, sizeof(int32_t) * kAbortInts,
                          hipMemcpyDeviceToHost, ls));
   HIP_TRY(hipEventRecord(g.done_ev, ls));
   return OK;
@@ -955,7 +955,7 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
   }
   // Distances, abort records and tables of every group, on its stream.
   std::vector<std::vector<int32_t>> gd(ng);
-  std::vector<std::array<int32_t, 8>> aborted(ng);
+  std::vector<std::array<int32_t, kAbortInts>> aborted(ng);
   for (int k = 0; k < ng; ++k) {
     Group& g = batch_[k];
     hipStream_t s = lane_stream(ng > 1 ? k : -1);
@@ -967,7 +967,7 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     HIP_TRY(hipMemcpyAsync(gd[k].data(), g.dist.p, sizeof(int32_t) * g.npairs,
                            hipMemcpyDeviceToHost, s));
     if (g.nxl > 0)
-      HIP_TRY(hipMemcpyAsync(aborted[k].data(), g.abort.p, sizeof(int32_t) * 8,
+      HIP_TRY(hipMemcpyAsync(aborted[k].data(), g.abort.p, sizeof(int32_t) * kAbortInts,
                              hipMemcpyDeviceToHost, s));
     if (td_out) {
       for (int32_t j = 0; j < g.npairs; ++j) {
@@ -1016,6 +1016,8 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     out.tables += g.st.tables;
     out.levels = std::max(out.levels, g.st.levels);
     out.launches += g.st.launches;
+    out.xl_stale += aborted[k][8];
+    out.xl_waits += aborted[k][9];
   }
   out.dp_ms = dp_ms;
   out.total_ms =
@@ -1089,7 +1091,7 @@ int Context::run_batch_noalloc(int32_t npairs, const int32_t* xs, const int32_t*
     return st;
   }
   hipStream_t s = lane_stream(-1);
-  int32_t rec[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int32_t rec[kAbortInts] = {};
   launch_gather_dist(na_.pairs.as<PairMeta>(), na_.td.as<int32_t>(), npairs,
                      na_.dist.as<int32_t>(), s);
   HIP_TRY(hipGetLastError());

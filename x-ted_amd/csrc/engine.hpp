@@ -34,6 +34,9 @@ struct BatchStats {
   int64_t tables = 0;      // keyroot-pair tables
   int32_t levels = 0;
   int32_t launches = 0;    // forest kernel launches
+  // Column-blocked launches: prefetched cross-workgroup reads that were
+  // stale and had to be polled, and pipeline checks that had to wait.
+  int64_t xl_stale = 0, xl_waits = 0;
 };
 
 // A fixed device region that buffers are carved from in order (no

@@ -295,13 +295,16 @@ __device__ __forceinline__ int32_t xl_wait(const unsigned long long* p, unsigned
 }
 
 // Arena words of one column-blocked task: nb-1 runs of carries (one per
-// row, each run padded to whole 64-byte lines), then for every saved row and
-// destination block a line-padded run of kds export words (by depth). A
+// row, each run padded to whole 128-byte L2 lines), then for every saved row
+// and destination block a line-padded run of kds export words (by depth). A
 // line thus only holds words its producer writes together, in order.
 struct XLGeom {
   int32_t nrows, nb, nsaved, kds;
 };
-__device__ __forceinline__ int32_t xl_pad(int32_t w) { return (w + 7) & ~7; }
+constexpr int32_t kXLLineWords = 16;  // 8-byte words per 128-byte L2 line
+__device__ __forceinline__ int32_t xl_pad(int32_t w) {
+  return (w + kXLLineWords - 1) & ~(kXLLineWords - 1);
+}
 __device__ __forceinline__ XLGeom xl_geom(const PoolView& P, const PairMeta& pm, uint64_t task,
                                           int bw) {
   const TreeMeta tx = P.trees[pm.tx], ty = P.trees[pm.ty];
@@ -342,7 +345,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // (The column-blocked form also rings its imports, and the cost form's
   // longer row body has to unroll S times: at most 8 slots for both.)
   constexpr int S0 = C >= 16 ? 2 : C >= 8 ? 4 : C >= 4 ? 8 : 16;
-  constexpr int S = (kXL || kCost) && S0 > 8 ? 8 : S0;
+  constexpr int S = kXL && S0 > 4 ? 4 : kCost && S0 > 8 ? 8 : S0;
   constexpr int PD = S - 1;  // rows of TD prefetched
   constexpr int BW = 64 * W * C;                    // columns of one block
   constexpr int KQ = kXL ? kMaxImports / 64 : 1;    // import entries per lane
@@ -407,6 +410,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   }
   int32_t nb = 1;
   bool seq = false;  // one workgroup sweeps every block of the table
+  // (kXL) Wave 0's count of prefetched reads that were stale (carry or
+  // import rows that had to be polled again) and of pipeline checks that had
+  // to wait, added to the launch's totals at the end.
+  int32_t n_stale = 0, n_wait = 0;
   if constexpr (kXL) {
     nb = (ncols + BW - 1) / BW;
     seq = kSeqForm && nrows <= A.xl_seq_rows;
@@ -602,9 +609,9 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // again. Reads not wanted count as valid.
   unsigned long long* xw = A.xl_words;
   int64_t cin_off = 0, cout_off = 0, exp_off = 0;
-  const int32_t kdsp = (kds + 7) & ~7;
+  const int32_t kdsp = xl_pad(kds);
   if constexpr (kXL) {
-    const int64_t cstride = (nrows + 7) & ~7;
+    const int64_t cstride = xl_pad(nrows);
     xw += A.xl_off[tix];
     cin_off = (int64_t)(blk - 1) * cstride;
     cout_off = (int64_t)blk * cstride;
@@ -648,14 +655,17 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // The poll for row k's check is read 4 rows ahead (tw), so its round trip
   // overlaps those rows.
   unsigned long long tw = 0;
-  auto xl_throttle_row = [&](int32_t k) { return min((k + PD + kXLEvery - 1) | 7, nrows - 1); };
+  auto xl_throttle_row = [&](int32_t k) {
+    return min((k + PD + kXLEvery - 1) | (kXLLineWords - 1), nrows - 1);
+  };
   auto xl_throttle_word = [&](int32_t k) { return xw + cin_off + xl_throttle_row(k); };
   auto xl_throttle = [&](int32_t k, bool early) {
     if constexpr (kXL) {
       if (blk > 0 && !seq) {
         const int32_t t = xl_throttle_row(k);
-        (void)xl_wait(xl_throttle_word(k), early ? tw : xl_load(xl_throttle_word(k)), true,
-                      A.xl_epoch, A.xl_abort, ticket, t);
+        const unsigned long long w0 = early ? tw : xl_load(xl_throttle_word(k));
+        n_wait += (uint32_t)(w0 >> 32) != A.xl_epoch;
+        (void)xl_wait(xl_throttle_word(k), w0, true, A.xl_epoch, A.xl_abort, ticket, t);
       }
     }
   };
@@ -798,9 +808,12 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       int32_t carry = kCost ? frc + (hc - hq) : (rmax - r) + 1;
       if constexpr (kXL) {
         if (blk > 0) {
-          carry = (crok >> u) & 1u ? crv[u]
-                                   : xl_wait(xw + cin_off + k, 0ull, true, A.xl_epoch,
-                                             A.xl_abort, ticket, k);
+          if ((crok >> u) & 1u) {
+            carry = crv[u];
+          } else {
+            ++n_stale;
+            carry = xl_wait(xw + cin_off + k, 0ull, true, A.xl_epoch, A.xl_abort, ticket, k);
+          }
         }
       }
       if constexpr (W > 1) {
@@ -866,6 +879,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
             for (int q2 = 0; q2 < KQ; ++q2) {
               const bool stale = impl[q2] && !((irok[u] >> q2) & 1u);
               const bool any = __ballot(stale) != 0;  // uniform
+              n_stale += any;
               const int32_t v =
                   any ? xl_wait(xword(max(irid[u], 0), blk, lane + 64 * q2), 0ull, stale,
                                 A.xl_epoch, A.xl_abort, ticket, (1 << 24) | k)
@@ -892,6 +906,12 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     }
   }
   } while (kSeqForm && seq && ++blk < nb);  // blocks of a table swept by one workgroup
+  if constexpr (kXL) {
+    if (threadIdx.x == 0 && (n_stale | n_wait)) {
+      atomicAdd(A.xl_abort + 8, n_stale);
+      atomicAdd(A.xl_abort + 9, n_wait);
+    }
+  }
 #undef TCOL
 }
 

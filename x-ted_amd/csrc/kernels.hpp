@@ -74,7 +74,8 @@ struct PlanArgs {
   uint64_t* tasks;
   TaskDesc* descs;             // one per task, same order
   unsigned int* tickets;       // per bucket: column-blocked launch counters
-  int32_t* abort_flag;         // 8 ints: column-blocked waits gave up, and what they waited for
+  int32_t* abort_flag;         // kAbortInts: column-blocked waits gave up, what they waited for,
+                               // and their stale-read / wait counts
 };
 // Blocks a pair occupies in the plan kernels (same formula on both sides).
 inline int64_t plan_blocks(int32_t Kx, int32_t Ky, int32_t chunk) {
@@ -109,6 +110,9 @@ struct LeafArgs {
 // (see forest.hip), so it must finish before any table of the group runs.
 void launch_leaf(const LeafArgs& a, int64_t ntiles, hipStream_t s);
 
+// Ints of a group's column-blocked record (DPArgs::xl_abort).
+constexpr int kAbortInts = 16;
+
 // Ints of padding the TD arena keeps before and after every table, so the
 // row-scan kernels can issue whole-group loads without bounds branches.
 constexpr int kTdPad = 8;
@@ -125,7 +129,8 @@ struct DPArgs {
   unsigned long long* xl_words;  // cross-workgroup words of every such task
   const int64_t* xl_off;         // this launch's tasks' offsets into xl_words
   unsigned int* xl_ticket;       // (task, block) counter of this launch, zeroed
-  int32_t* xl_abort;             // [0] raised when a wait gives up, [1..6] what it waited for
+  int32_t* xl_abort;             // [0] raised when a wait gives up, [1..6] what it waited for,
+                                 // [8] stale prefetched reads, [9] throttle checks that waited
   uint32_t xl_epoch;             // tag of this launch's words (never 0)
   int32_t xl_nbmax;              // workgroups per task in the grid
   int32_t xl_seq_rows;           // tables this short: one workgroup sweeps all blocks

@@ -363,7 +363,7 @@ __global__ void plan_clear_kernel(PlanArgs a, int nb) {
     a.maxlds[i] = 0;
     a.tickets[i] = 0;
   }
-  if (i < 8) a.abort_flag[i] = 0;
+  if (i < kAbortInts) a.abort_flag[i] = 0;
 }
 
 void launch_plan(const PlanArgs& a, int64_t nblocks, hipStream_t s) {

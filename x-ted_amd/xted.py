@@ -123,7 +123,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("plan_ms", ctypes.c_double), ("dp_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("cells", ctypes.c_int64),
                 ("tables", ctypes.c_int64), ("levels", ctypes.c_int32),
-                ("launches", ctypes.c_int32)]
+                ("launches", ctypes.c_int32), ("xl_stale_reads", ctypes.c_int64),
+                ("xl_waits", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
