@@ -178,12 +178,15 @@ def xl_pair_bench(reps=3):
         ids = ctx.pool_add([x, y])
         xs, ys = ids[:1].copy(), ids[1:].copy()
         ctx.batch(xs, ys)  # warm
-        ctx.set_profile(True)
         runs = []
         for _ in range(reps):
             t = time.perf_counter()
             d, _, st = ctx.batch(xs, ys)
             runs.append((time.perf_counter() - t, st, int(d[0])))
+        # Then one run with every launch event-timed (per-class device
+        # time, the root table's time per row); it is not the timed one.
+        ctx.set_profile(True)
+        _, _, pst = ctx.batch(xs, ys)
         prof = ctx.class_profile()
         launches = ctx.launch_profile()
     wall, st, dist = sorted(runs, key=lambda r: r[0])[len(runs) // 2]
@@ -206,6 +209,8 @@ def xl_pair_bench(reps=3):
             "frac_of_hbm_peak": round(BYTES_PER_CELL * st.cells / wall / 1e9 / HBM_PEAK_GBS, 4),
             "cells": st.cells, "tables": st.tables, "levels": st.levels,
             "plan_ms": round(st.plan_ms, 2), "dp_ms": round(st.dp_ms, 2),
+            "profiled_ms": round(pst.total_ms, 2),
+            "xl_stale_reads": st.xl_stale_reads, "xl_waits": st.xl_waits,
             "column_blocked_ms": xl, "classes": classes,
             "root_table": {"class": xted.class_name(root["cls"]), "ms": round(root["ms"], 3),
                            "rows": 20000, "us_per_row": round(root["ms"] * 1e3 / 20000, 3)},
@@ -384,8 +389,9 @@ def main():
                     help="skip the batch under a non-unit cost model")
     ap.add_argument("--serve-only", action="store_true",
                     help="only the serving leg (for profiling it alone)")
-    ap.add_argument("--serve-rate2", type=float, default=0.0,
-                    help="also serve a second stream at this offered rate (near saturation)")
+    ap.add_argument("--serve-rate2", type=float, default=150.0,
+                    help="also serve a second stream at this offered rate (near saturation: "
+                         "about 110-150 pairs/s complete on this pool; 0 skips it)")
     ap.add_argument("--serve-seconds2", type=float, default=5.0)
     ap.add_argument("--dump-launches", default="",
                     help="write the per-launch records of the event-timed steps (1 and 4 level "
@@ -472,8 +478,8 @@ def main():
         # traces for the dominant kernel's per-dispatch duration.
         ctx.set_level_streams(1)
         _, prof1, _ = timed(max(1, args.profile_steps), True, "profiled step (1 stream)", dist)
-        roof1, _ = dominant_roofline(prof1)
-        print(json.dumps({"roofline": roof1}), flush=True)
+        roof1, classes1 = dominant_roofline(prof1)
+        print(json.dumps({"roofline": roof1, "classes_1_stream": classes1}), flush=True)
         return
     # The timed steps: levels replayed from a captured graph unless
     # --no-graphs. Then timed regions with every forest launch bracketed by

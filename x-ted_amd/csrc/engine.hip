@@ -70,8 +70,12 @@ int DevBuf::grow(size_t bytes, bool keep, hipStream_t s) {
   }
   if (p) {
     if (keep && cap) HIP_TRY(hipMemcpyAsync(np, p, cap, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    HIP_TRY(hipFree(p));
+    if (retire && !keep) {
+      retire->push_back(p);
+    } else {
+      HIP_TRY(hipStreamSynchronize(s));
+      HIP_TRY(hipFree(p));
+    }
   }
   p = np;
   cap = ncap;
@@ -783,16 +787,26 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
   return OK;
 }
 
+int Context::reserve_pinned(Group& g, int32_t n) {
+  if (g.pin_cap >= n) return OK;
+  if (g.h_dist_pin) HIP_TRY(hipHostFree(g.h_dist_pin));
+  g.h_dist_pin = nullptr;
+  g.pin_cap = 0;
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.h_dist_pin),
+                        sizeof(int32_t) * ((size_t)n + kAbortInts), hipHostMallocDefault));
+  g.pin_cap = n;
+  return OK;
+}
+
+int Context::release_retired() {
+  for (void* p : retired_) HIP_TRY(hipFree(p));
+  retired_.clear();
+  return OK;
+}
+
 int Context::finish_async(Group& g, int lane) {
   hipStream_t ls = lane_stream(lane);
-  if (g.pin_cap < g.npairs) {
-    if (g.h_dist_pin) HIP_TRY(hipHostFree(g.h_dist_pin));
-    g.h_dist_pin = nullptr;
-    g.pin_cap = 0;
-    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.h_dist_pin),
-                          sizeof(int32_t) * (g.npairs + kAbortInts), hipHostMallocDefault));
-    g.pin_cap = g.npairs;
-  }
+  ST_TRY(reserve_pinned(g, g.npairs));
   if (!g.done_ev) HIP_TRY(hipEventCreateWithFlags(&g.done_ev, hipEventDisableTiming));
   launch_gather_dist(g.pairs.as<PairMeta>(), g.td.as<int32_t>(), g.npairs, g.dist.as<int32_t>(),
                      ls);

@@ -49,12 +49,15 @@ struct Arena {
 // Growable device buffer. grow() keeps contents when `keep` is set. A buffer
 // bound to an arena never allocates: grow() carves the next piece of the
 // arena (contents are not kept) or fails with ERR_TOO_LARGE, and release()
-// only forgets the piece.
+// only forgets the piece. A buffer with a retire list hands its old memory
+// to that list when it grows instead of freeing it (hipFree waits for the
+// whole device): the owner frees the list when nothing runs.
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
   bool fine = false;  // fine-grained allocation (hipDeviceMallocFinegrained)
   Arena* arena = nullptr;
+  std::vector<void*>* retire = nullptr;
   int grow(size_t bytes, bool keep, hipStream_t s);
   void release();
   template <class T>
@@ -206,6 +209,13 @@ class Context {
   // Enqueues the distance gather of a group whose levels are all launched;
   // poll() returns ERR_BUSY until it has landed (wait = block for it).
   int finish_async(Group& g, int lane = -1);
+  // Pinned host memory for the distances (and the column-blocked record) of
+  // up to n pairs of g.
+  int reserve_pinned(Group& g, int32_t n);
+  // Device memory that grown group buffers handed over (DevBuf::retire),
+  // and its release; the caller holds mutex() and nothing of theirs runs.
+  std::vector<void*>* retire_list() { return &retired_; }
+  int release_retired();
   hipStream_t lane_stream(int lane) const;  // where a lane's ticks end
   int lane_streams(int lane, hipStream_t* out) const;
   int poll(Group& g, bool wait, int32_t* dist);
@@ -272,6 +282,7 @@ class Context {
   DevBuf prep_scratch_, prep_off_;
   Group batch_[kMaxLevelStreams];  // the groups run_batch reuses
   Group na_;                        // run_batch_noalloc's group, bound to na_arena_
+  std::vector<void*> retired_;      // see retire_list()
   Arena na_arena_;                  // this device's fixed array (kNoAllocBytes)
   int64_t noalloc_bytes(int32_t npairs, const int32_t* xs, const int32_t* ys) const;
   uint32_t xl_epoch_ = 0;  // tag of the latest column-blocked launch

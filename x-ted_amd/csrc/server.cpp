@@ -17,6 +17,9 @@ Server::~Server() {
   }
   cv_work_.notify_all();
   if (worker_.joinable()) worker_.join();
+  std::lock_guard<std::mutex> lk(ctx_->mutex());
+  (void)ctx_->drain();
+  (void)ctx_->release_retired();
 }
 
 int Server::submit(int32_t x, int32_t y, uint64_t tag) {
@@ -111,7 +114,16 @@ void Server::admit(int lane) {
     g = std::move(spare_.back());
     spare_.pop_back();
   } else {
+    // A new group: buffers that outgrow themselves go to the context's
+    // retire list (hipFree would wait for every lane's work), and the
+    // pinned result buffer holds the largest group from the start.
     g.reset(new Group());
+    g->for_each_buf([this](DevBuf& b) { b.retire = ctx_->retire_list(); });
+    std::lock_guard<std::mutex> lk(ctx_->mutex());
+    if (ctx_->reserve_pinned(*g, max_batch_) != OK) {
+      publish(batch, ERR_OUT_OF_MEMORY, nullptr, Clock::now());
+      return;
+    }
   }
   auto fill = [&](std::vector<int32_t>& xs, std::vector<int32_t>& ys) {
     xs.resize(batch.size());
@@ -251,6 +263,14 @@ void Server::loop() {
       spare_.push_back(std::move(a.g));
       completing_.erase(completing_.begin() + i);
       progressed = true;
+    }
+    // Nothing in flight: memory that grown groups handed over can be freed
+    // now without holding up any lane.
+    bool empty = completing_.empty();
+    for (const Lane& L : lanes_) empty &= L.active.empty();
+    if (empty) {
+      std::lock_guard<std::mutex> lk(ctx_->mutex());
+      if (!ctx_->retire_list()->empty()) (void)ctx_->release_retired();
     }
     // Both lanes two ticks ahead of the device and nothing landed: give the
     // device a moment (new queries still wake the wait).

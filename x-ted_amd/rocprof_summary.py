@@ -5,10 +5,15 @@ under profiles/.
             per kernel: dispatches, average and total duration; with the
             bench JSON line, the algorithmic bytes per launch of each class
             (4 bytes of TD per interior cell) and the rate they imply.
-  counters: python x-ted_amd/rocprof_summary.py counters <sq.csv> <lds.csv>
+  counters: python x-ted_amd/rocprof_summary.py counters <sq.csv> <lds.csv> [--occ occ.csv] [--valu valu.csv]
             per kernel from the SQ counter runs: waves, VALU / LDS / VMEM
             instructions per wave, VALU-active and waiting shares of wave
-            cycles, LDS bank-conflict share of LDS-active cycles.
+            cycles, LDS bank-conflict share of LDS-active cycles; with the
+            derived-counter runs, mean occupancy per CU, VALU busy and SIMD
+            utilisation.
+  resources: python x-ted_amd/rocprof_summary.py resources [csrc/forest.hip]
+            registers, scratch, static LDS and waves per SIMD of every
+            kernel as the compiler reports them for gfx950.
   traffic:  python x-ted_amd/rocprof_summary.py traffic KERNEL <fetch.csv> <write.csv>
                 <copy_fetch.csv> <copy_write.csv> --copy-bytes N
             FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the dispatches of
@@ -18,6 +23,7 @@ under profiles/.
 import argparse
 import csv
 import json
+import os
 import sys
 from collections import defaultdict
 
@@ -94,6 +100,62 @@ def kernels(path, bench=None):
     return "\n".join(out)
 
 
+def resources(src, hipcc="/opt/rocm/bin/hipcc", arch="gfx950"):
+    """Registers, scratch, LDS and occupancy of every kernel of a HIP source
+    as the compiler reports them (-Rpass-analysis=kernel-resource-usage),
+    compiled with the library's device flags. rocprofv3's kernel-trace
+    register columns count allocation granules and its LDS column leaves out
+    dynamic LDS; these are the compiler's own counts."""
+    import re
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    cmd = [hipcc, f"--offload-arch={arch}", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+           "-Xclang", "-packed-fp32-ops", "-mllvm", "-pragma-unroll-threshold=1000000",
+           "--cuda-device-only", "-c", src, "-o", os.devnull,
+           "-Rpass-analysis=kernel-resource-usage"]
+    r = subprocess.run(cmd, cwd=here, capture_output=True, text=True, check=True)
+    rows, cur = [], None
+    for line in r.stderr.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            cur = {"name": m.group(1)}
+            rows.append(cur)
+            continue
+        for key, pat in (("vgpr", r" VGPRs: (\d+)"), ("agpr", r"AGPRs: (\d+)"),
+                         ("sgpr", r"TotalSGPRs: (\d+)"), ("scratch_B", r"ScratchSize \[bytes/lane\]: (\d+)"),
+                         ("occupancy", r"Occupancy \[waves/SIMD\]: (\d+)"),
+                         ("lds_static_B", r"LDS Size \[bytes/block\]: (\d+)")):
+            m = re.search(pat, line)
+            if m and cur is not None:
+                cur[key] = int(m.group(1))
+
+    def pretty(mangled):
+        m = re.search(r"rowscan_kernelILi(\d+)ELi(\d+)ELb([01])ELb([01])", mangled)
+        if m:
+            w, c, xl, cost = (int(v) for v in m.groups())
+            base = f"xl_bw{64 * w * c}" if xl else f"rowscan_w{w}_c{c}"
+            return base + ("_cost" if cost else "")
+        m = re.search(r"N4xted\d+(\w+?)E", mangled) or re.search(r"4xted(\d+)(\w+)", mangled)
+        for name in ("leaf_kernel", "tiny_kernel", "plan_kernel", "tree_prep_kernel",
+                     "cost_prefix_kernel", "xl_layout_kernel", "xl_zero_kernel",
+                     "plan_scan_kernel", "plan_clear_kernel", "gather_dist_kernel"):
+            if name in mangled:
+                if name == "tiny_kernel":
+                    return "tiny_kernel_cost" if "ILb1E" in mangled else "tiny_kernel"
+                if name == "plan_kernel":
+                    return "plan_kernel<true>" if "ILb1E" in mangled else "plan_kernel<false>"
+                return name
+        return mangled
+    head = (f"{'kernel':<24}{'vgpr':>6}{'agpr':>6}{'sgpr':>6}{'scratch_B':>10}"
+            f"{'waves/SIMD':>11}{'lds_static_B':>13}")
+    out = [head]
+    for r in rows:
+        out.append(f"{pretty(r['name']):<24}{r.get('vgpr', 0):>6}{r.get('agpr', 0):>6}"
+                   f"{r.get('sgpr', 0):>6}{r.get('scratch_B', 0):>10}{r.get('occupancy', 0):>11}"
+                   f"{r.get('lds_static_B', 0):>13}")
+    return "\n".join(out)
+
+
 def _sums(path):
     per = defaultdict(lambda: defaultdict(float))
     with open(path) as f:
@@ -102,18 +164,46 @@ def _sums(path):
     return per
 
 
-def counters(sq_path, lds_path):
+def _weighted(path):
+    """Per kernel and derived counter: the average over dispatches weighted
+    by each dispatch's duration (for ratios such as MeanOccupancyPerCU)."""
+    acc = defaultdict(lambda: defaultdict(lambda: [0.0, 0.0]))
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            a = acc[short(r["Kernel_Name"])][r["Counter_Name"]]
+            a[0] += float(r["Counter_Value"]) * d
+            a[1] += d
+    return {k: {c: v[0] / max(v[1], 1) for c, v in cs.items()} for k, cs in acc.items()}
+
+
+def counters(sq_path, lds_path, occ_path=None, valu_path=None):
+    """SQ / LDS counter sums per kernel; with the occupancy and VALU-busy
+    runs, also the duration-weighted MeanOccupancyPerCU (waves resident per
+    CU), VALUBusy (% of SIMD cycles issuing VALU) and SIMD_UTILIZATION
+    (share of time CUs hold any wave)."""
     sq, lds = _sums(sq_path), _sums(lds_path)
-    out = [f"{'kernel':<22}{'waves':>12}{'valu/wave':>11}{'lds/wave':>10}{'vmem/wave':>10}"
-           f"{'valu_active':>12}{'waiting':>9}{'lds_conflict':>13}"]
+    occ = _weighted(occ_path) if occ_path else {}
+    valu = _weighted(valu_path) if valu_path else {}
+    head = (f"{'kernel':<22}{'waves':>12}{'valu/wave':>11}{'lds/wave':>10}{'vmem/wave':>10}"
+            f"{'valu_active':>12}{'waiting':>9}{'lds_conflict':>13}")
+    if occ or valu:
+        head += f"{'occ/CU':>8}{'VALUBusy%':>10}{'SIMD_util':>10}"
+    out = [head]
     for k in sorted(sq, key=lambda k: -sq[k]["SQ_BUSY_CYCLES"]):
         a, b = sq[k], lds.get(k, {})
         w = max(a["SQ_WAVES"], 1)
         cyc = max(a["SQ_WAVE_CYCLES"], 1)
-        out.append(f"{k:<22}{a['SQ_WAVES']:>12.0f}{a['SQ_INSTS_VALU'] / w:>11.0f}"
-                   f"{a['SQ_INSTS_LDS'] / w:>10.0f}{a['SQ_INSTS_VMEM_RD'] / w:>10.0f}"
-                   f"{a['SQ_ACTIVE_INST_VALU'] / cyc:>12.3f}{a['SQ_WAIT_INST_ANY'] / cyc:>9.3f}"
-                   f"{b.get('SQ_LDS_BANK_CONFLICT', 0) / max(b.get('SQ_LDS_IDX_ACTIVE', 0), 1):>13.3f}")
+        line = (f"{k:<22}{a['SQ_WAVES']:>12.0f}{a['SQ_INSTS_VALU'] / w:>11.0f}"
+                f"{a['SQ_INSTS_LDS'] / w:>10.0f}{a['SQ_INSTS_VMEM_RD'] / w:>10.0f}"
+                f"{a['SQ_ACTIVE_INST_VALU'] / cyc:>12.3f}{a['SQ_WAIT_INST_ANY'] / cyc:>9.3f}"
+                f"{b.get('SQ_LDS_BANK_CONFLICT', 0) / max(b.get('SQ_LDS_IDX_ACTIVE', 0), 1):>13.3f}")
+        if occ or valu:
+            o, v = occ.get(k, {}), valu.get(k, {})
+            line += (f"{o.get('MeanOccupancyPerCU', float('nan')):>8.2f}"
+                     f"{v.get('VALUBusy', float('nan')):>10.1f}"
+                     f"{v.get('SIMD_UTILIZATION', float('nan')):>10.3f}")
+        out.append(line)
     return "\n".join(out)
 
 
@@ -156,6 +246,11 @@ def main():
     c = sub.add_parser("counters")
     c.add_argument("sq")
     c.add_argument("lds")
+    c.add_argument("--occ", help="run with --pmc MeanOccupancyPerCU")
+    c.add_argument("--valu", help="run with --pmc VALUBusy SIMD_UTILIZATION")
+    rs = sub.add_parser("resources")
+    rs.add_argument("source", nargs="?", default="csrc/forest.hip",
+                    help="HIP source, relative to x-ted_amd/")
     t = sub.add_parser("traffic")
     t.add_argument("kernel")
     t.add_argument("fetch")
@@ -164,10 +259,12 @@ def main():
     t.add_argument("copy_write")
     t.add_argument("--copy-bytes", type=int, required=True)
     a = ap.parse_args()
-    if a.cmd == "kernels":
+    if a.cmd == "resources":
+        print(resources(a.source))
+    elif a.cmd == "kernels":
         print(kernels(a.trace, a.bench))
     elif a.cmd == "counters":
-        print(counters(a.sq, a.lds))
+        print(counters(a.sq, a.lds, a.occ, a.valu))
     else:
         print(traffic(a.kernel, a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes))
 
