@@ -306,3 +306,70 @@ def test_server_across_a_cost_model_change():
         status, dist = second[100 + k]
         assert status == xted.OK
         assert dist == oracle.ted_costs(trees[a], trees[b], c2, threads=4, want_td=False)[0]
+
+
+def comb_tree(depth, alphabet, seed):
+    """Each spine node's first child the next spine node, its second a leaf
+    (preorder: the whole deeper spine comes before the leaf): a right-to-left
+    sweep holds every such leaf's value until it reaches its previous
+    sibling, so the cost form of the leaf lines needs a stack `depth` deep."""
+    parent = [-1]
+    spine = [0]
+    for _ in range(depth):
+        parent.append(spine[-1])
+        spine.append(len(parent) - 1)
+    # the leaves, second children, in preorder after each spine node's subtree
+    for node in reversed(spine[:-1]):
+        parent.append(node)
+    # renumber into preorder: children of a spine node are (next spine, leaf)
+    kids = {}
+    for c, pa in enumerate(parent[1:], 1):
+        kids.setdefault(pa, []).append(c)
+    order, st = [], [0]
+    while st:
+        v = st.pop()
+        order.append(v)
+        st.extend(reversed(kids.get(v, [])))
+    nid = {v: i for i, v in enumerate(order)}
+    pre = np.full(len(parent), -1, np.int32)
+    for v in range(1, len(parent)):
+        pre[nid[v]] = nid[parent[v]]
+    rng = np.random.default_rng(seed)
+    return pre, rng.integers(0, alphabet, len(parent)).astype(np.int32)
+
+
+@pytest.mark.parametrize("depth", [40, 127, 128, 129, 200])
+def test_cost_leaf_lines_stack_limit(cost_ctx, depth):
+    """Pairs whose trees need a line stack up to the limit (128) take the
+    cost form of the leaf kernel; deeper ones plan their leaf tables as row
+    scans. Full tables against the oracle on both sides of the limit, with
+    the other tree random."""
+    rng = np.random.default_rng(depth)
+    costs = rand_costs(rng, 9)
+    cost_ctx.set_costs(costs)
+    x = comb_tree(depth, 9, depth)
+    y = gen(xted.GEN_RRT, 150, 9, 931, depth)
+    assert_same(cost_ctx, x, y, costs)
+    assert_same(cost_ctx, y, x, costs)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_cost_leaf_form_on_and_off(cost_ctx, seed):
+    """The same batch with the leaf lines in the cost form of the closed form
+    and with every leaf table planned as a row scan or tiny table: equal
+    distances, both equal to the oracle's."""
+    rng = np.random.default_rng(40 + seed)
+    costs = rand_costs(rng, 12, zeros=seed % 2 == 1)
+    cost_ctx.set_costs(costs)
+    kinds = (xted.GEN_RRT, xted.GEN_DEEP, xted.GEN_SHALLOW)
+    trees = [gen_any(kinds[k % 3], 30 + 37 * k, 12, 932 + seed, k) for k in range(10)]
+    ids = cost_ctx.pool_add(trees)
+    xs, ys = ids[[0, 1, 2, 3, 4, 5, 6, 7]], ids[[9, 8, 7, 6, 5, 4, 3, 2]]
+    d_on, _, _ = cost_ctx.batch(xs, ys)
+    cost_ctx.force_class(xted.CLASS_AUTO, lane_enable=False)
+    d_off, _, _ = cost_ctx.batch(xs, ys)
+    assert np.array_equal(d_on, d_off)
+    for k in range(len(xs)):
+        od, _ = oracle.ted_costs(trees[int(xs[k]) - int(ids[0])], trees[int(ys[k]) - int(ids[0])],
+                                 costs, want_td=False)
+        assert d_on[k] == od

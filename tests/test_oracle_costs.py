@@ -87,3 +87,64 @@ def test_costs_empty_trees_sum_the_other_side():
     empty = (np.empty(0, np.int32), np.empty(0, np.int32))
     assert oracle.ted_costs(x, empty, (dele, ins, ren))[0] == int(dele[x[1]].sum())
     assert oracle.ted_costs(empty, x, (dele, ins, ren))[0] == int(ins[x[1]].sum())
+
+
+def _subtree_min_sweep(parent, orl, w):
+    """Subtree minima m[c] = min(w[k] for k in [c, orl[c]]) by one right-to-left
+    sweep with a stack of pending sibling values: R[c] = min over c's subtree
+    and its later siblings' subtrees; a node pops its next sibling's R and
+    pushes its own R when it has a previous sibling. Returns (m, deepest
+    stack)."""
+    n = len(parent)
+    m = [0] * n
+    stack, deepest, r_prev = [], 0, 0
+    for c in range(n - 1, -1, -1):
+        internal = orl[c] > c
+        has_prev = c > 0 and parent[c] != c - 1
+        has_next = c > 0 and orl[c] < orl[parent[c]]
+        mc = min(w[c], r_prev) if internal else w[c]
+        rc = min(mc, stack.pop()) if has_next else mc
+        if has_prev:
+            stack.append(rc)
+            deepest = max(deepest, len(stack))
+        r_prev = rc
+        m[c] = mc
+    assert not stack
+    return m, deepest
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_cost_leaf_lines_closed_form(seed):
+    """Tables with a one-node side under a cost model: for a leaf v of x and
+    any node c of y, TD[v][c] = sum ins(sub_y c) + min(del(v), min over k in
+    sub_y(c) of ren(v, k) - ins(k)) (v deleted, or mapped onto one node k of
+    the subtree, the rest inserted), and symmetrically for a leaf of y. Every
+    leaf row and column of the oracle's full table must follow it; the
+    subtree minima also come out of the single right-to-left sweep the device
+    uses, checked here against their definition."""
+    rng = np.random.default_rng(100 + seed)
+    a = int(rng.integers(2, 8))
+    kinds = ("rrt", "path", "star")
+    x = rand_tree(rng, int(rng.integers(1, 45)), a, kinds[seed % 3])
+    y = rand_tree(rng, int(rng.integers(1, 45)), a, kinds[(seed // 3) % 3])
+    dele, ins, ren = rand_costs(rng, a)
+    if seed % 4 == 1:
+        dele[:] = 0  # free deletions
+    _, td = oracle.ted_costs(x, y, (dele, ins, ren))
+    for (rp, rl), (tp, tl), line_of_x in ((x, y, True), (y, x, False)):
+        rp, rl, tp, tl = (np.asarray(v) for v in (rp, rl, tp, tl))
+        if len(rp) == 0 or len(tp) == 0:
+            continue
+        ro, to = np.asarray(oracle.orl(rp)), np.asarray(oracle.orl(tp))
+        # costs of the other tree's nodes when v is mapped or not
+        keep = ins if line_of_x else dele
+        for v in np.flatnonzero(ro == np.arange(len(ro))):
+            w = [int((ren[rl[v], tl[k]] if line_of_x else ren[tl[k], rl[v]]) - keep[tl[k]])
+                 for k in range(len(tp))]
+            m, _ = _subtree_min_sweep(tp, to, w)
+            dv = int(dele[rl[v]] if line_of_x else ins[rl[v]])
+            for c in range(len(tp)):
+                assert m[c] == min(w[c:to[c] + 1])
+                want = int(keep[tl[c:to[c] + 1]].sum()) + min(dv, m[c])
+                got = td[v, c] if line_of_x else td[c, v]
+                assert got == want, (seed, line_of_x, v, c)

@@ -246,6 +246,7 @@ int Context::pool_add(int32_t ntrees, const int32_t* sizes, const int32_t* paren
     compute_orl(parents + in_off, n, orl.data());
     ti[t].n = n;
     ti[t].ksum = keyroot_size_sum(orl.data(), keyroots_desc(parents + in_off, orl.data(), n));
+    ti[t].line_stack = line_stack_depth(parents + in_off, orl.data(), n);
     ti[t].maxlab = *std::max_element(labels + in_off, labels + in_off + n);
     ti[t].minlab = *std::min_element(labels + in_off, labels + in_off + n);
     if (cost_alphabet_ > 0 && (ti[t].minlab < 0 || ti[t].maxlab >= cost_alphabet_)) {
@@ -463,15 +464,18 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   g.leaf_tables = g.leaf_cells = 0;
   int32_t nlevels = 0;
   g.maxn = 1;
-  // The closed-form leaf tables assume unit costs: with a cost model the
-  // tables of a leaf keyroot are planned like the others.
-  const int32_t le = cost_alphabet_ > 0 ? 0 : lane_enable;
+  // Pairs whose leaf lines are in closed form (per pair under a cost model,
+  // whose form needs both trees within its stack); the others plan the tables
+  // of their leaf keyroots like the rest.
+  std::vector<uint8_t> leaf(npairs, 0);
   for (int32_t i = 0; i < npairs; ++i) {
     const int32_t x = xs[i], y = ys[i];
     if (x < 0 || y < 0 || x >= (int32_t)meta_.size() || y >= (int32_t)meta_.size()) {
       set_error("run_batch: tree id out of range in pair " + std::to_string(i));
       return ERR_NOT_FOUND;
     }
+    const bool le = leaf_form(x, y);
+    leaf[i] = le ? 1 : 0;
     const TreeMeta &a = meta_[x], &b = meta_[y];
     const int64_t mn = (int64_t)a.n * b.n;
     // Far values sit in 16-bit LDS slots: every forest value (at most
@@ -538,6 +542,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   ST_TRY(g.td.grow((size_t)g.td_bytes, false, ps));
   ST_TRY(g.dist.grow(sizeof(int32_t) * npairs, false, ps));
   ST_TRY(g.lines_d.grow(sizeof(int64_t) * (npairs + 1), false, ps));
+  ST_TRY(g.leaf_d.grow((size_t)npairs, false, ps));
   if (level_align) ST_TRY(g.loff_d.grow(sizeof(int32_t) * npairs, false, ps));
   HIP_TRY(hipMemcpyAsync(g.pairs.p, g.pm.data(), sizeof(PairMeta) * npairs,
                          hipMemcpyHostToDevice, ps));
@@ -546,6 +551,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   HIP_TRY(hipMemcpyAsync(g.lines_d.p, g.lines.data(), sizeof(int64_t) * (npairs + 1),
                          hipMemcpyHostToDevice, ps));
   HIP_TRY(hipMemcpyAsync(g.tail_lv.p, g.h_tail.data(), nlevels, hipMemcpyHostToDevice, ps));
+  HIP_TRY(hipMemcpyAsync(g.leaf_d.p, leaf.data(), (size_t)npairs, hipMemcpyHostToDevice, ps));
   if (level_align)
     HIP_TRY(hipMemcpyAsync(g.loff_d.p, g.loff.data(), sizeof(int32_t) * npairs,
                            hipMemcpyHostToDevice, ps));
@@ -559,12 +565,8 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   if (lane < 0 && !profile && g.lines[npairs] > 0) {
     hipStream_t ls = lane_stream(lane);
     HIP_TRY(hipStreamWaitEvent(ls, g.ev_plan0, 0));
-    LeafArgs la;
-    la.pool = view();
-    la.pairs = g.pairs.as<PairMeta>();
-    la.tile_start = g.lines_d.as<int64_t>();
-    la.npairs = npairs;
-    la.td = g.td.as<int32_t>();
+    LeafArgs la = leaf_args(g);
+    la.npairs = npairs;  // g.npairs is set once the plan is complete
     launch_leaf(la, g.lines[npairs], ls);
     HIP_TRY(hipGetLastError());
     ++st.launches;
@@ -584,7 +586,8 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   pa.tail_level = g.tail_lv.as<uint8_t>();
   pa.force_class = force_class;
   pa.bands = size_bands;
-  pa.lane_enable = le;
+  pa.lane_enable = lane_enable;
+  pa.pair_leaf = g.leaf_d.as<uint8_t>();
   pa.counts = g.counts.as<unsigned long long>();
   pa.cells = g.cells.as<unsigned long long>();
   pa.cursor = g.cursor.as<unsigned long long>();
@@ -731,13 +734,7 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
     hipStream_t s = i < nleaf ? sl[0] : sl[(i - nleaf) % ns];
     if (prof) HIP_TRY(hipEventRecord(prof_ev_[2 * prof->size()], s));
     if (w.cls == kClassLane) {
-      LeafArgs la;
-      la.pool = view();
-      la.pairs = g.pairs.as<PairMeta>();
-      la.tile_start = g.lines_d.as<int64_t>();
-      la.npairs = g.npairs;
-      la.td = g.td.as<int32_t>();
-      launch_leaf(la, (int64_t)w.cnt, s);
+      launch_leaf(leaf_args(g), (int64_t)w.cnt, s);
     } else {
       DPArgs da;
       da.pool = view();
@@ -854,10 +851,31 @@ int Context::drain() {
   return OK;
 }
 
+bool Context::leaf_form(int32_t x, int32_t y) const {
+  if (!lane_enable) return false;
+  return cost_alphabet_ == 0 ||
+         (info_[x].line_stack <= kCostLeafStack && info_[y].line_stack <= kCostLeafStack);
+}
+
+LeafArgs Context::leaf_args(const Group& g) const {
+  LeafArgs la;
+  la.pool = view();
+  la.pairs = g.pairs.as<PairMeta>();
+  la.tile_start = g.lines_d.as<int64_t>();
+  la.npairs = g.npairs;
+  la.td = g.td.as<int32_t>();
+  la.del = cdel_.as<int32_t>();
+  la.ins = cins_.as<int32_t>();
+  la.ren = cren_.as<int32_t>();
+  la.alphabet = cost_alphabet_;
+  la.costs = cost_alphabet_ > 0 ? 1 : 0;
+  return la;
+}
+
 int64_t Context::pair_device_bytes(int32_t x, int32_t y) const {
   const TreeMeta &a = meta_[x], &b = meta_[y];
-  const bool leaf_form = cost_alphabet_ == 0 && lane_enable;
-  const int64_t ka = leaf_form ? a.K - a.K1 : a.K, kb = leaf_form ? b.K - b.K1 : b.K;
+  const bool lf = leaf_form(x, y);
+  const int64_t ka = lf ? a.K - a.K1 : a.K, kb = lf ? b.K - b.K1 : b.K;
   return 8 * (int64_t)a.n * b.n + 3 * kTdPad * (int64_t)sizeof(int32_t) +
          (int64_t)(sizeof(uint64_t) + sizeof(TaskDesc)) * ka * kb;
 }
@@ -1044,11 +1062,11 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
 // the arena does (the column-blocked words, sized on the device, come on top).
 int64_t Context::noalloc_bytes(int32_t npairs, const int32_t* xs, const int32_t* ys) const {
   auto r = [](int64_t b) { return (b + 255) & ~int64_t{255}; };
-  const bool le = cost_alphabet_ == 0 && lane_enable;
   int64_t td = 0, planned = 0;
   int32_t nlevels = 0;
   for (int32_t i = 0; i < npairs; ++i) {
     const TreeMeta &a = meta_[xs[i]], &b = meta_[ys[i]];
+    const bool le = leaf_form(xs[i], ys[i]);
     td += 2 * (int64_t)a.n * b.n + 3 * kTdPad;
     planned += (int64_t)(le ? a.K - a.K1 : a.K) * (le ? b.K - b.K1 : b.K);
     nlevels = std::max(nlevels, a.kdmax + b.kdmax + 1);
@@ -1057,7 +1075,7 @@ int64_t Context::noalloc_bytes(int32_t npairs, const int32_t* xs, const int32_t*
   planned = std::max<int64_t>(planned, 1);
   return r(sizeof(PairMeta) * npairs) + r(nlevels) + 2 * r(8 * (int64_t)(npairs + 1)) +
          3 * r(8 * nb) + 2 * r(4 * nb) + r(32) + r(8 * planned) + r(64 * planned) + r(4 * td) +
-         r(4 * (int64_t)npairs) + (level_align ? r(4 * (int64_t)npairs) : 0);
+         r(4 * (int64_t)npairs) + (level_align ? r(4 * (int64_t)npairs) : 0) + r(npairs);
 }
 
 int Context::run_batch_noalloc(int32_t npairs, const int32_t* xs, const int32_t* ys,

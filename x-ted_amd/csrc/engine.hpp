@@ -13,6 +13,8 @@
 
 namespace xted {
 
+struct LeafArgs;  // kernels.hpp
+
 // Thread-local message for the last failing call on this thread.
 void set_error(const std::string& msg);
 const char* last_error();
@@ -90,7 +92,7 @@ struct Group {
   std::vector<int32_t> h_maxlds;
   std::vector<uint8_t> h_tail;
   DevBuf pairs, blk, counts, cells, cursor, maxlds, tickets, tasks, descs, td, dist, lines_d,
-      tail_lv, xl_off, xl_total, loff_d;
+      tail_lv, xl_off, xl_total, loff_d, leaf_d;
   DevBuf abort, xl_words;
   // finish_async: distances then the abort flag land in pinned memory
   int32_t* h_dist_pin = nullptr;
@@ -114,7 +116,7 @@ struct Group {
   void for_each_buf(F f) {
     for (DevBuf* b : {&pairs, &blk, &counts, &cells, &cursor, &maxlds, &tickets, &abort, &tasks,
                       &descs, &td, &dist, &lines_d, &tail_lv, &xl_off, &xl_total, &xl_words,
-                      &loff_d})
+                      &loff_d, &leaf_d})
       f(*b);
   }
 };
@@ -141,6 +143,7 @@ struct HostTreeInfo {
   int64_t ksum = 0;    // S = sum of keyroot subtree sizes
   int32_t minlab = 0, maxlab = 0;  // smallest and largest label id
   int64_t sdel = 0, sins = 0;  // cost mode: deleting / inserting the whole tree
+  int32_t line_stack = 0;      // line_stack_depth of the tree
 };
 
 class Context {
@@ -285,6 +288,11 @@ class Context {
   std::vector<void*> retired_;      // see retire_list()
   Arena na_arena_;                  // this device's fixed array (kNoAllocBytes)
   int64_t noalloc_bytes(int32_t npairs, const int32_t* xs, const int32_t* ys) const;
+  // Whether the leaf lines of pair (x, y) are written in closed form (the
+  // leaf kernel) rather than planned as tables: lane_enable, and under a
+  // cost model both trees within the cost form's stack.
+  bool leaf_form(int32_t x, int32_t y) const;
+  LeafArgs leaf_args(const Group& g) const;
   uint32_t xl_epoch_ = 0;  // tag of the latest column-blocked launch
   std::vector<hipEvent_t> prof_ev_;
   int grow_pool(int64_t nodes);

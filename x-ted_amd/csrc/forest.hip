@@ -171,11 +171,154 @@ __global__ __launch_bounds__(256) void leaf_kernel(LeafArgs A, int64_t ntiles, i
   }
 }
 
+// ---------------------------------------------------------------------------
+// Leaf-keyroot lines under a cost model (the closed form in kernels.hpp). For
+// line v (a leaf of R) over the other tree T: with H the cost of T's nodes
+// that are not mapped (inserted when v is x's, deleted when it is y's), w(k)
+// = ren(v, k) - H(k) (ren(k, v) for v of y) and d(v) the cost of v alone,
+//   TD(v, c) = H(sub c) + min(d(v), m(c)),   m(c) = min over k in sub(c) of w(k).
+// One right-to-left sweep over T gives every m(c): with R(c) the minimum over
+// c's subtree and the subtrees of its later siblings,
+//   m(c) = w(c) for a leaf, min(w(c), R(c+1)) otherwise;
+//   R(c) = min(m(c), R(next sibling)) when c has one, m(c) otherwise.
+// Between a node and its previous sibling the sweep passes only the
+// previous sibling's subtree, so R of a next sibling waits on a stack: pushed
+// at a node that has a previous sibling, popped at a node that has a next
+// one. All lines of a tile sweep the same T, so every push and pop is the
+// same for all lanes and lane l keeps its line's entries in column l of the
+// LDS stack, the top entry in a register (so a push followed by a pop does
+// not wait on LDS). One wave per tile of 64 consecutive nodes of R (the unit
+// kernel's tiles); lane l is node v0 + l. Per step of 64 positions the
+// metadata of position top - i sits in lane i (read per position with
+// readlane), the rename values are loaded a chunk of 16 positions ahead,
+// and the 64 x 64 values (below 2^16) are written from a 16-bit LDS tile as
+// the unit kernel writes them: row segments of every line, then the column
+// layout.
+// ---------------------------------------------------------------------------
+constexpr int kCostLeafChunk = 16;
+__global__ __launch_bounds__(64) void leaf_cost_kernel(LeafArgs A, int64_t ntiles, int64_t base) {
+  __shared__ int16_t stk[kCostLeafStack][64];
+  __shared__ uint16_t tile[kLeafTile][kLeafTile + 2];
+  const int64_t tix = base + blockIdx.x;
+  if (tix >= ntiles) return;
+  const int lane = threadIdx.x & 63;
+  int lo = 0, hi = A.npairs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (A.tile_start[mid] <= tix) lo = mid;
+    else hi = mid - 1;
+  }
+  const PairMeta pm = A.pairs[lo];
+  const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
+  const int32_t q = (int32_t)(tix - A.tile_start[lo]);
+  const int32_t qx = (tx.n + kLeafTile - 1) / kLeafTile;
+  const bool of_x = q < qx;  // nodes of x: the lines are rows of TD
+  const TreeMeta& R = of_x ? tx : ty;
+  const TreeMeta& T = of_x ? ty : tx;
+  const int32_t v0 = (of_x ? q : q - qx) * kLeafTile;
+  const int32_t vl = v0 + lane;
+  const int32_t vc = min(vl, R.n - 1);
+  const bool is_line = vl < R.n && A.pool.orl[R.node_off + vc] == vc &&
+                       A.pool.kdn[R.node_off + vc] >= 0;
+  const unsigned long long lmask = __ballot(is_line);
+  if (lmask == 0) return;  // uniform (one wave)
+  const int32_t labv = A.pool.label[R.node_off + vc];
+  // d(v): deleting v (a node of x) or inserting it (a node of y).
+  const int32_t* Pv = (of_x ? A.pool.pdel : A.pool.pins) + R.node_off;
+  const int32_t dv = Pv[vc + 1] - Pv[vc];
+  // T's unmapped nodes are inserted (v of x) or deleted (v of y).
+  const int32_t* PT = (of_x ? A.pool.pins : A.pool.pdel) + T.node_off;
+  const int32_t* HT = of_x ? A.ins : A.del;
+  const int32_t* __restrict__ labT = A.pool.label + T.node_off;
+  const int32_t* __restrict__ orlT = A.pool.orl + T.node_off;
+  const int32_t* __restrict__ parT = A.pool.parent + T.node_off;
+  const int32_t* __restrict__ kdnT = A.pool.kdn + T.node_off;
+  // ren(v, k) = ren[labv * a + lab k] for v of x, ren[lab k * a + labv] for v of y.
+  const int32_t* const renv = A.ren + (of_x ? (int64_t)labv * A.alphabet : labv);
+  const int32_t rstride = of_x ? 1 : A.alphabet;
+  int32_t* const rows = A.td + (of_x ? pm.td_off : pm.tdt_off);  // [v][c], stride T.n
+  int32_t* const cols = A.td + (of_x ? pm.tdt_off : pm.td_off);  // [c][v], stride R.n
+  const bool col_ok = vl < R.n;
+  // Stack entries [0, sp-1) in LDS, entry sp-1 in tos.
+  int32_t sp = 0, rprev = 0, tos = 0;
+  for (int32_t top = T.n - 1; top >= 0; top -= kLeafTile) {
+    // Lane i: position p = top - i (positions past 0 repeat position 0).
+    const int32_t p = top - lane;
+    const int32_t pc = max(p, 0);
+    const int32_t lab = labT[pc], o = orlT[pc], par = parT[pc];
+    const int32_t opar = par >= 0 ? orlT[par] : o;
+    // bit 0: internal node; bit 1: has a previous sibling (push); bit 2: has
+    // a next sibling (pop); bit 3: a leaf keyroot of T (its cells are written
+    // by T's own tiles in the column layout)
+    const int32_t flags = (o > p ? 1 : 0) | (p > 0 && par != p - 1 ? 2 : 0) |
+                          (p > 0 && o < opar ? 4 : 0) | (o == p && kdnT[pc] >= 0 ? 8 : 0);
+    const int32_t hsub = PT[o + 1] - PT[pc];  // H(sub p)
+    const int32_t hk = HT[lab];               // H(p)
+    const int32_t cnt = min(kLeafTile, top + 1);
+    int32_t wq[2][kCostLeafChunk];
+    auto issue = [&](int chunk, int32_t* dst) {
+#pragma unroll
+      for (int j = 0; j < kCostLeafChunk; ++j)
+        dst[j] = renv[(int64_t)__builtin_amdgcn_readlane(lab, chunk * kCostLeafChunk + j) * rstride];
+    };
+    issue(0, wq[0]);
+#pragma unroll
+    for (int ch = 0; ch < kLeafTile / kCostLeafChunk; ++ch) {
+      if (ch * kCostLeafChunk >= cnt) break;  // uniform
+      if (ch + 1 < kLeafTile / kCostLeafChunk) issue(ch + 1, wq[(ch + 1) & 1]);
+#pragma unroll
+      for (int j = 0; j < kCostLeafChunk; ++j) {
+        const int i = ch * kCostLeafChunk + j;
+        if (i >= cnt) break;  // uniform
+        const int32_t f = __builtin_amdgcn_readlane(flags, i);
+        const int32_t w = wq[ch & 1][j] - __builtin_amdgcn_readlane(hk, i);
+        const int32_t m = (f & 1) ? min(w, rprev) : w;
+        int32_t r = m;
+        if (f & 4) {  // pop
+          r = min(m, tos);
+          sp = max(sp - 1, 0);
+          if (sp > 0) tos = stk[sp - 1][lane];
+        }
+        if ((f & 2) && sp < kCostLeafStack) {  // push
+          if (sp > 0) stk[sp - 1][lane] = (int16_t)tos;
+          tos = r;
+          ++sp;
+        }
+        rprev = r;
+        tile[i][lane] = (uint16_t)(__builtin_amdgcn_readlane(hsub, i) + min(dv, m));
+      }
+    }
+    __syncthreads();
+    // Row segments: line l's values at positions top - i, i in lanes.
+    const bool pos_ok = lane < cnt;
+    unsigned long long lm = lmask;
+    while (lm) {
+      const int l = __ffsll((long long)lm) - 1;
+      lm &= lm - 1;
+      if (pos_ok) rows[(int64_t)(v0 + l) * T.n + (top - lane)] = (int32_t)tile[lane][l];
+    }
+    // Column layout: position top - i across the tile's 64 nodes.
+    const unsigned long long skip = __ballot(!pos_ok || (flags & 8));
+    for (int i = 0; i < cnt; ++i) {
+      if (!((skip >> i) & 1ull) && col_ok)
+        cols[(int64_t)(top - i) * R.n + vl] = (int32_t)tile[i][lane];
+    }
+    __syncthreads();
+  }
+}
+
 // A grid's threads (gridDim.x * blockDim.x) must stay below 2^32: launches
 // over more work items than kGridThreads / threads-per-group go in pieces.
 constexpr int64_t kGridThreads = int64_t{1} << 31;
 
 void launch_leaf(const LeafArgs& a, int64_t ntiles, hipStream_t s) {
+  if (a.costs) {
+    constexpr int64_t kChunk = kGridThreads / 64;
+    for (int64_t base = 0; base < ntiles; base += kChunk)
+      hipLaunchKernelGGL(leaf_cost_kernel, dim3((unsigned)std::min(kChunk, ntiles - base)),
+                         dim3(64), 0, s, a, ntiles, base);
+    return;
+  }
   constexpr int64_t kChunk = kGridThreads / 256;
   for (int64_t base = 0; base < ntiles; base += kChunk)
     hipLaunchKernelGGL(leaf_kernel, dim3((unsigned)std::min(kChunk, ntiles - base)), dim3(256), 0,

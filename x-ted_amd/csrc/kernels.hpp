@@ -67,6 +67,7 @@ struct PlanArgs {
   int32_t force_class;         // -1 = automatic; else row-scan tables go to this class if they fit
   int32_t bands;               // 0: every table in band 0 (plan order within a launch)
   int32_t lane_enable;         // 0 = single-row tables use the row-scan kernels
+  const uint8_t* pair_leaf;    // per pair: its lines are in closed form (null: lane_enable)
   unsigned long long* counts;  // per bucket (bucket_of: level, class, band)
   unsigned long long* cells;   // per bucket: interior cells
   unsigned long long* cursor;  // scatter cursors (bucket offsets)
@@ -104,7 +105,25 @@ struct LeafArgs {
   const int64_t* tile_start;
   int32_t npairs;
   int32_t* td;
+  // Cost model (costs != 0): per-label delete / insert costs and the rename
+  // matrix ren[a * alphabet + b]; the pool's pdel / pins hold their prefix
+  // sums over each tree's nodes.
+  const int32_t* del;
+  const int32_t* ins;
+  const int32_t* ren;
+  int32_t alphabet;
+  int32_t costs;
 };
+// Under a cost model the lines take the cost form of the closed form: for a
+// leaf v of x and a node c of y,
+//   TD[v][c] = ins(sub_y c) + min(del(v), min over k in sub_y(c) of
+//              ren(v, k) - ins(k)),
+// (v deleted, or mapped onto one node k of the subtree; everything else
+// inserted), and symmetrically for a leaf of y. Its subtree minima take one
+// right-to-left sweep over the other tree with a stack of pending sibling
+// values as deep as the tree's kLineStack measure (at most kCostLeafStack;
+// pairs with deeper trees plan their leaf tables as row scans).
+constexpr int kCostLeafStack = 128;
 // One workgroup per tile of 64 consecutive nodes of one side of a pair (its
 // leaf keyroots are the lines). Writes cells of other tables' nodes as well
 // (see forest.hip), so it must finish before any table of the group runs.

@@ -30,7 +30,8 @@ __device__ __forceinline__ int find_pair(const int64_t* blk_start, int npairs, i
 
 // Per-table classification shared by both passes.
 __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
-                                         const TreeMeta& ty, int32_t loff, int k, int l, int* bucket,
+                                         const TreeMeta& ty, int32_t loff, bool leaf_form, int k,
+                                         int l, int* bucket,
                                          int* lds, uint32_t* i0, uint32_t* j0,
                                          unsigned long long* cells) {
   const int64_t kx = tx.node_off + k, ly = ty.node_off + l;
@@ -47,7 +48,7 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
   int64_t need = (int64_t)slots * (leaves_c + 1) * 2;
   // With the leaf kernel off, single-row tables take the row-scan forms.
   int cls = table_class(a, b, need, a <= b ? kdx : kdy, a <= b ? kdy : kdx, leaves_c,
-                        A.lane_enable != 0, rule, &need);
+                        leaf_form, rule, &need);
   // A forced column-blocked form takes every multi-row table it can hold.
   if (A.force_class >= kFirstXLClass && A.force_class < kFirstXLClass + kNumXLForms && cls >= 1 &&
       cls != kClassXL) {
@@ -159,8 +160,9 @@ __device__ __forceinline__ PlanChunk plan_chunk(const PlanArgs& A, const TreeMet
   // With the leaf kernel on, tables with a leaf keyroot are not planned here
   // (the leaf kernel writes them): only the non-leaf keyroots [K1, K) of the
   // sorted lists take part.
-  c.kbase = A.lane_enable ? tx.K1 : 0;
-  c.lbase = A.lane_enable ? ty.K1 : 0;
+  const bool le = A.pair_leaf ? A.pair_leaf[p] != 0 : A.lane_enable != 0;
+  c.kbase = le ? tx.K1 : 0;
+  c.lbase = le ? ty.K1 : 0;
   c.Kx = tx.K - c.kbase;
   const int Ky = ty.K - c.lbase;
   const int lper = max(1, A.lchunk_tables / max(c.Kx, 1));  // same as plan_blocks()
@@ -278,6 +280,7 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(PlanArgs A) {
   const TreeMeta tx = A.pool.trees[pm.tx], ty = A.pool.trees[pm.ty];
   const PlanChunk ch = plan_chunk(A, tx, ty, p);
   const int32_t loff = A.level_off ? A.level_off[p] : 0;
+  const bool leaf_form = A.pair_leaf ? A.pair_leaf[p] != 0 : A.lane_enable != 0;
   ht_clear(T);
   __syncthreads();
   auto sweep = [&](auto body) {
@@ -289,7 +292,7 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(PlanArgs A) {
       unsigned long long cells = 0;
       if (active) {
         const int l = ch.lbase + ch.l_lo + (int)(idx / ch.Kx), k = ch.kbase + (int)(idx % ch.Kx);
-        classify(A, tx, ty, loff, k, l, &bucket, &lds, &i0, &j0, &cells);
+        classify(A, tx, ty, loff, leaf_form, k, l, &bucket, &lds, &i0, &j0, &cells);
       }
       body(active, bucket, lds, i0, j0, cells);
     }

@@ -92,6 +92,15 @@ void keyroot_height(const int32_t* parent, const int32_t* orl, int32_t n, int32_
   }
 }
 
+int32_t line_stack_depth(const int32_t* parent, const int32_t* orl, int32_t n) {
+  int32_t sp = 0, deepest = 0;
+  for (int32_t c = n - 1; c > 0; --c) {
+    if (orl[c] < orl[parent[c]]) --sp;  // has a next sibling: its value is popped
+    if (parent[c] != c - 1) deepest = std::max(deepest, ++sp);  // has a previous one
+  }
+  return deepest;
+}
+
 int64_t keyroot_size_sum(const int32_t* orl, const std::vector<int32_t>& kr) {
   int64_t s = 0;
   for (int32_t k : kr) s += (int64_t)orl[k] - k + 1;

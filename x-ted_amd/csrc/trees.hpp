@@ -54,6 +54,10 @@ std::vector<int32_t> keyroots_desc(const int32_t* parent, const int32_t* orl, in
 void keyroot_height(const int32_t* parent, const int32_t* orl, int32_t n, int32_t* kd);
 // S = sum over keyroots of |subtree(k)|; cells of a pair are S1*S2.
 int64_t keyroot_size_sum(const int32_t* orl, const std::vector<int32_t>& kr);
+// Deepest stack of pending sibling values in a right-to-left sweep over the
+// tree (one entry pushed at each node with a previous sibling, popped at its
+// previous sibling): what the cost form of the leaf lines needs per line.
+int32_t line_stack_depth(const int32_t* parent, const int32_t* orl, int32_t n);
 
 // ---- seeded generators -----------------------------------------------------
 enum GenKind : int { GEN_RRT = 0, GEN_DEEP = 1, GEN_SHALLOW = 2 };
