@@ -428,7 +428,7 @@ def main():
     xs, ys = ids[0::2].copy(), ids[1::2].copy()
     import autotune
     if args.autotune:
-        rule, _ = autotune.tune(ctx, xs, ys, log=log)
+        rule, _, _ = autotune.tune(ctx, xs, ys, log=log)
     else:
         rule = autotune.load()
         if args.rule:

@@ -1,13 +1,26 @@
-"""Autotuner for the granularity cut-overs.
+"""Autotuner for the granularity cut-overs and the batch schedule.
 
 The reference fixes its granularity thresholds at 15 / 18,750 / 25,250,000
 cells (X-TED_GPU/parallel_X_TED.cu:382) from a cost model fitted to one GPU
-(supplement, section 2). Here the class rule has four knobs
-(include/xted.h, xted_set_tuning): slot bytes per wave before a table gets
-more waves, the most columns per lane, and a minimum wave count above a
-column threshold. This module times a workload under a grid of rules on the
-device it runs on and keeps the fastest. Every candidate must reproduce the
-distances of the default rule exactly, or it is rejected.
+(supplement, section 2). Here the class rule has knobs (include/xted.h,
+xted_set_tuning / xted_set_tail_rule / xted_set_level_align /
+xted_set_size_bands): slot bytes per wave before a table gets more waves,
+the most columns per lane, a latency rule for the last levels, where each
+pair's levels sit in the batch and the order of a launch's tables. This
+module times a workload under a staged grid of rules on the device it runs
+on:
+
+  1. the schedule: level alignment x size bands;
+  2. the tail rule;
+  3. a coarse grid of slot bytes per wave x columns per lane;
+  4. a fine grid of slot bytes per wave (1 KiB steps) around the best.
+
+The workload is the 4096-pair uneven batch's generator on a seed stream of
+its own (not the one bench.py times). The rule starts as the default; a
+candidate replaces the current best only when it is faster by more than
+the spread measured over repeated steps of the default rule, so noise
+cannot pick a rule. Every candidate must reproduce the default rule's
+distances exactly, or the run stops.
 
     python x-ted_amd/autotune.py [--pairs N] [--out x-ted_amd/tuning.json]
 """
@@ -26,12 +39,16 @@ import xted  # noqa: E402
 
 DEFAULT = {"lds_per_wave": 10 * 1024, "max_cpl": 8, "min_waves": 1, "wide_cols": 1 << 30,
            "tail_pairs": 32, "level_align": 0, "size_bands": 1}
-# Searched in stages: the tail rule first, then the throughput rule with the
-# best tail setting.
+# Seed stream of the tuning workload (bench.py times stream 3).
+SEED_TUNE = 1003
+COARSE_LDS = [6 * 1024, 10 * 1024, 14 * 1024, 20 * 1024]
 STAGES = [
-    {"tail_pairs": [0, 8, 32, 128, 512]},
-    {"lds_per_wave": [5 * 1024, 10 * 1024, 20 * 1024], "max_cpl": [4, 8]},
+    {"level_align": [0, 1, 2], "size_bands": [0, 1]},
+    {"tail_pairs": [0, 8, 32]},
+    {"lds_per_wave": COARSE_LDS, "max_cpl": [4, 8, 16]},
 ]
+FINE_STEP = 1024
+FINE_SPAN = 3  # steps either side of the coarse best
 TUNING_FILE = os.path.join(HERE, "tuning.json")
 
 
@@ -52,46 +69,62 @@ def load(path=TUNING_FILE):
         return dict(DEFAULT)
 
 
-def time_rule(ctx, xs, ys, rule, reps=2):
+def steps(ctx, xs, ys, rule, reps):
+    """Per-step times of `reps` batches under `rule` (after a warm-up) and
+    the distances."""
     apply(ctx, rule)
-    dist, _, _ = ctx.batch(xs, ys)  # warm
-    best = float("inf")
+    dist, _, _ = ctx.batch(xs, ys)  # warm (plans, captures the graph)
+    times = []
     for _ in range(reps):
         t = time.perf_counter()
         d, _, _ = ctx.batch(xs, ys)
-        best = min(best, time.perf_counter() - t)
+        times.append(time.perf_counter() - t)
         if not np.array_equal(d, dist):
             raise RuntimeError("distances changed between runs")
-    return best, dist
+    return times, dist
 
 
-def tune(ctx, xs, ys, stages=STAGES, log=print):
-    """Returns (best_rule, results) over the staged grid; the default rule's
-    distances are the reference every candidate must match."""
-    base_t, base_d = time_rule(ctx, xs, ys, DEFAULT)
+def tune(ctx, xs, ys, stages=STAGES, reps=3, log=print):
+    """Returns (best_rule, results, spread): the staged search described in
+    the module docstring. results lists (rule, best step time)."""
+    base_times, base_d = steps(ctx, xs, ys, DEFAULT, 6)
+    spread = max(base_times) - min(base_times)
+    base_t = min(base_times)
     results = [(dict(DEFAULT), base_t)]
-    log(f"default {DEFAULT}: {base_t * 1e3:.1f} ms")
+    log(f"default {DEFAULT}: {base_t * 1e3:.1f} ms, step-to-step spread {spread * 1e3:.2f} ms")
     best, best_t = dict(DEFAULT), base_t
+
+    def trial(rule):
+        nonlocal best, best_t
+        if any(rule == r for r, _ in results):
+            return
+        try:
+            times, d = steps(ctx, xs, ys, rule, reps)
+        except xted.XtedError as e:
+            log(f"{rule}: rejected ({e})")
+            return
+        if not np.array_equal(d, base_d):
+            raise RuntimeError(f"rule {rule} changed a distance")
+        t = min(times)
+        results.append((rule, t))
+        won = t < best_t - spread
+        log(f"{rule}: {t * 1e3:.1f} ms{' (new best)' if won else ''}")
+        if won:
+            best, best_t = rule, t
+
     for grid in stages:
         keys = list(grid)
         start = dict(best)
         for vals in itertools.product(*(grid[k] for k in keys)):
-            rule = dict(start, **dict(zip(keys, vals)))
-            if any(rule == r for r, _ in results):
-                continue
-            try:
-                t, d = time_rule(ctx, xs, ys, rule)
-            except xted.XtedError as e:
-                log(f"{rule}: rejected ({e})")
-                continue
-            if not np.array_equal(d, base_d):
-                raise RuntimeError(f"rule {rule} changed a distance")
-            results.append((rule, t))
-            log(f"{rule}: {t * 1e3:.1f} ms")
-            if t < best_t:
-                best, best_t = rule, t
+            trial(dict(start, **dict(zip(keys, vals))))
+    # Fine steps of slot bytes per wave around the coarse best.
+    start = dict(best)
+    for k in range(-FINE_SPAN, FINE_SPAN + 1):
+        lds = start["lds_per_wave"] + k * FINE_STEP
+        if lds >= 2 * 1024:
+            trial(dict(start, lds_per_wave=lds))
     apply(ctx, best)
-    return best, results
+    return best, results, spread
 
 
 def main():
@@ -101,15 +134,18 @@ def main():
     args = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
     from bench import batch_trees
-    trees = batch_trees(args.pairs)
+    trees = batch_trees(args.pairs, cfg=SEED_TUNE)
     with xted.Context(0) as ctx:
         ids = ctx.pool_add(trees)
-        best, results = tune(ctx, ids[0::2].copy(), ids[1::2].copy())
-    out = {"rule": best, "workload": f"batch{args.pairs}_uneven_n100-1000_deep50_shallow50",
+        best, results, spread = tune(ctx, ids[0::2].copy(), ids[1::2].copy())
+    out = {"rule": best,
+           "workload": f"batch{args.pairs}_uneven_n100-1000_deep50_shallow50_alphabet200 "
+                       f"(seed stream {SEED_TUNE})",
+           "spread_ms": round(spread * 1e3, 2),
            "results_ms": [{"rule": r, "ms": round(t * 1e3, 2)} for r, t in results]}
     with open(args.out, "w") as f:
         json.dump(out, f, indent=1)
-    print(json.dumps({"best": best}))
+    print(json.dumps({"best": best, "spread_ms": round(spread * 1e3, 2)}))
 
 
 if __name__ == "__main__":
