@@ -217,6 +217,43 @@ def xl_pair_bench(reps=3):
             "checked": "distance against the oracle (tests/golden/xl_expected.json)"}
 
 
+def row_latency(forms=((1000, (-1, 5, 9, 13, 17, 21, 29, 30, 31, 32)),
+                        (4000, (-1, 28, 29, 30)))):
+    """Time per row of one large table: the root table of a random recursive
+    tree pair (alphabet 200), which runs alone in the last level, under each
+    forced form (-1: the form the rule picks). One level stream; the best of
+    three event-timed runs."""
+    out = []
+    for n, classes in forms:
+        x = xted.generate(xted.GEN_RRT, n, 200, xted.derive_seed(901, n))
+        y = xted.generate(xted.GEN_RRT, n, 200, xted.derive_seed(902, n))
+        with xted.Context(0) as ctx:
+            ids = ctx.pool_add([x, y])
+            xs, ys = ids[:1].copy(), ids[1:].copy()
+            ctx.set_level_streams(1)
+            ref = None
+            for cls in classes:
+                ctx.force_class(cls)
+                d, _, _ = ctx.batch(xs, ys)
+                ref = int(d[0]) if ref is None else ref
+                if int(d[0]) != ref:
+                    raise SystemExit(f"row latency: forced class {cls} changed the distance")
+                ctx.set_profile(True)
+                best, root_cls = None, None
+                for _ in range(3):
+                    ctx.batch(xs, ys)
+                    recs = ctx.launch_profile()
+                    last = max(r["level"] for r in recs)
+                    root = max((r for r in recs if r["level"] == last), key=lambda r: r["ms"])
+                    if best is None or root["ms"] < best:
+                        best, root_cls = root["ms"], root["cls"]
+                ctx.set_profile(False)
+                out.append({"n": n, "forced": cls, "form": xted.class_name(root_cls),
+                            "root_ms": round(best, 3), "us_per_row": round(best * 1e3 / n, 3)})
+                log(f"row latency {out[-1]}")
+    return out
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -393,6 +430,8 @@ def main():
                     help="also serve a second stream at this offered rate (near saturation: "
                          "about 110-150 pairs/s complete on this pool; 0 skips it)")
     ap.add_argument("--serve-seconds2", type=float, default=5.0)
+    ap.add_argument("--row-latency", action="store_true",
+                    help="only the time per row of one large table under each forced form")
     ap.add_argument("--dump-launches", default="",
                     help="write the per-launch records of the event-timed steps (1 and 4 level "
                          "streams) to this JSON file")
@@ -402,6 +441,9 @@ def main():
     args = ap.parse_args()
     if args.calibrate_copy:
         copy_calibration(args.calibrate_copy)
+        return
+    if args.row_latency:
+        print(json.dumps({"row_latency": row_latency()}), flush=True)
         return
     if args.xl_only:
         print(json.dumps({"xl_pair": xl_pair_bench()}), flush=True)

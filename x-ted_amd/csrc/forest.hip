@@ -351,8 +351,9 @@ typedef int32_t int2u __attribute__((ext_vector_type(2), aligned(4)));
 //     For a block those columns are the far columns of the right paths that
 //     cross its right edge, one per nesting depth below the column keyroot,
 //     so they live in kd_c+1 import entries of the block's slot rows. The
-//     owner of f writes them at its row R to every block up to the one
-//     holding the path's top keyroot.
+//     owner of f writes each such value once, at its row R, into its own
+//     run of words for that row (one word per depth); every later block that
+//     imports it reads it there.
 // Words crossing workgroups are {epoch:32 | value:32}, written once per
 // launch with device-scope relaxed atomics; a reader polls until the epoch
 // matches, so no fence is needed and nothing is cleared between launches.
@@ -439,8 +440,9 @@ __device__ __forceinline__ int32_t xl_wait(const unsigned long long* p, unsigned
 
 // Arena words of one column-blocked task: nb-1 runs of carries (one per
 // row, each run padded to whole 128-byte L2 lines), then for every saved row
-// and destination block a line-padded run of kds export words (by depth). A
-// line thus only holds words its producer writes together, in order.
+// and source block a line-padded run of kds export words (by depth). A line
+// thus only holds words of one producer, and no block that reads a run
+// writes to it.
 struct XLGeom {
   int32_t nrows, nb, nsaved, kds;
 };
@@ -605,8 +607,8 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   //   woff: the index this column writes in a saved row (c-1 a leaf of
   //         subtree(c0)), or the sink entry;
   //   xinf: (kXL) for a written column whose right path's top keyroot lies
-  //         in a later block: that block << 16 | the path's depth index;
-  //         else -1;
+  //         in a later block: the path's depth index (its value is
+  //         exported); else -1;
   //   Fpm:  F[r+1][c] + 1 - t + T_g (the scan works on a[t] - t);
   //   negt: (kCost) the column's -Q(c).
   // Unit costs keep the values of group g shifted by T_g = TCOL(g, 0), this
@@ -663,20 +665,25 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
         const int32_t kdc = kdepC[c] - kdep_c0;
         if (imp) {
           foff[j] = wb + 1 + kdc;
-          xsh[kdc] = 1;
+          xsh[kdc] = (cmax - 2 - oc) / BW + 1;  // the block owning the far column, plus one
         }
         const int32_t u = rtopC[max(c - 1, c0)];
         const int32_t bu = (cmax - 1 - u) / BW;
-        xinf[j] = (writes && bu > blk) ? (bu << 16) | (kdepC[max(c - 1, c0)] - kdep_c0) : -1;
+        xinf[j] = (writes && bu > blk) ? kdepC[max(c - 1, c0)] - kdep_c0 : -1;
       }
     }
   }
-  // Which import entries this block uses (lane l holds depths l + 64q).
+  // Which import entries this block uses (lane l holds depths l + 64q), and
+  // the block each comes from.
   bool impl[KQ];
+  int32_t isrc[KQ];
   if constexpr (kXL) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < KQ; ++q) impl[q] = xsh[lane + 64 * q] != 0;
+    for (int q = 0; q < KQ; ++q) {
+      impl[q] = xsh[lane + 64 * q] != 0;
+      isrc[q] = max(xsh[lane + 64 * q] - 1, 0);
+    }
   }
 
   // TD row rr of this lane's columns into out[]. Loads are unconditional, so
@@ -762,7 +769,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   }
   int32_t crv[kXL ? S : 1], irv[kXL ? S : 1][KQ], irid[kXL ? S : 1];
   uint32_t crok = 0, irok[kXL ? S : 1];
-  // Export word of saved row rid for block bb at depth index kx.
+  // Export word of saved row rid written by block bb at depth index kx.
   auto xword = [&](int32_t rid, int32_t bb, int32_t kx) {
     return xw + exp_off + ((int64_t)rid * nb + bb) * kdsp + kx;
   };
@@ -781,7 +788,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       for (int q = 0; q < KQ; ++q) {
         bool okq = true;
         if (wave == 0 && impl[q] && rid >= 0 && k < nrows) {
-          const unsigned long long w = xl_peek(xword(rid, blk, lane + 64 * q));
+          const unsigned long long w = xl_peek(xword(rid, isrc[q], lane + 64 * q));
           irv[slot][q] = (int32_t)(uint32_t)w;
           okq = (uint32_t)(w >> 32) == A.xl_epoch;
         }
@@ -1024,7 +1031,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
               const bool any = __ballot(stale) != 0;  // uniform
               n_stale += any;
               const int32_t v =
-                  any ? xl_wait(xword(max(irid[u], 0), blk, lane + 64 * q2), 0ull, stale,
+                  any ? xl_wait(xword(max(irid[u], 0), isrc[q2], lane + 64 * q2), 0ull, stale,
                                 A.xl_epoch, A.xl_abort, ticket, (1 << 24) | k)
                       : 0;
               const int32_t val = stale ? v : irv[u][q2];
@@ -1034,11 +1041,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           const int32_t rid = __builtin_amdgcn_readlane(mrid, i);
 #pragma unroll
           for (int j = 0; j < C; ++j) {
-            if (xinf[j] >= 0) {
-              const int32_t kx = xinf[j] & 0xFFFF;
-              for (int32_t bb = blk + 1; bb <= (xinf[j] >> 16); ++bb)
-                xl_store(xword(rid, bb, kx), A.xl_epoch, h[j] - term(j), seq);
-            }
+            if (xinf[j] >= 0) xl_store(xword(rid, blk, xinf[j]), A.xl_epoch, h[j] - term(j), seq);
           }
         }
         // The next row may read this slot row at columns of other waves.
