@@ -184,6 +184,11 @@ int xted_set_tail_rule(xted_context* ctx, int32_t tail_pairs);
  * block by one workgroup (no waits between workgroups); taller ones run one
  * workgroup per block, chained per row. */
 int xted_set_xl_rule(xted_context* ctx, int32_t seq_rows);
+/* Widest column block (128 to 4096 columns; default 4096) that a taller
+ * column-blocked table may use: narrower blocks put more workgroups on one
+ * table. The reference's multi-block form uses 8 blocks of 1024 threads
+ * (X-TED_GPU/parallel_X_TED.cu:883-961). Results do not depend on it. */
+int xted_set_xl_width(xted_context* ctx, int32_t max_block_cols);
 /* Streams (1 to 4) that the launches of one level run on concurrently. The
  * reference gives each class kernel its own stream but waits for each
  * launch before the next (X-TED_GPU/parallel_X_TED.cu:433-438). Results do

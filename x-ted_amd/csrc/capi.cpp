@@ -458,6 +458,18 @@ int xted_force_class(xted_context* ctx, int32_t cls, int32_t lane_enable) {
   return OK;
 }
 
+int xted_set_xl_width(xted_context* ctx, int32_t max_block_cols) {
+  if (!ctx || max_block_cols < xl_bw(kNumXLForms - 1) || max_block_cols > xl_bw(0)) {
+    set_error("xted_set_xl_width: max_block_cols must be " + std::to_string(xl_bw(kNumXLForms - 1)) +
+              " to " + std::to_string(xl_bw(0)));
+    return ERR_INVALID_ARG;
+  }
+  int k = 0;
+  while (k + 1 < kNumXLForms && xl_bw(k) > max_block_cols) ++k;
+  ctx->ctx.rule.xl_first = k;
+  return OK;
+}
+
 int xted_set_xl_rule(xted_context* ctx, int32_t seq_rows) {
   if (!ctx || seq_rows < 0) {
     set_error("xted_set_xl_rule: invalid argument");

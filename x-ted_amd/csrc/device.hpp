@@ -161,6 +161,10 @@ struct ClassRule {
   int32_t max_cpl;       // most columns per lane before a table gets more waves
   int32_t xl_seq_rows = kXLSeqRows;  // column-blocked tables this short: one workgroup
   int32_t tiny = 1;                  // tables of <= kTinyCols columns take the tiny form
+  // Widest form a pipelined column-blocked table may use (0: 4096-column
+  // blocks, 1: 2048, 2: 1024, ...): narrower blocks put more workgroups on
+  // one table.
+  int32_t xl_first = 0;
 };
 
 XTED_HD inline int ilog2_ceil(int v) {
@@ -192,7 +196,8 @@ XTED_HD inline int table_class(int a, int b, int64_t lds_need, int kd_r, int kd_
   if (cols > 64 * cpl_cap * kMaxRowWaves || lds_need > rule.lds_max) {
     if (kd_c + 1 > kMaxImports) return kClassXL;
     // Short tables take the forms one workgroup sweeps.
-    for (int k = rows <= rule.xl_seq_rows ? kFirstSeqXLForm : 0; k < kNumXLForms; ++k) {
+    for (int k = rows <= rule.xl_seq_rows ? kFirstSeqXLForm : rule.xl_first; k < kNumXLForms;
+         ++k) {
       const int64_t n = xl_lds_need(kd_r, kd_c, leaves_c, xl_bw(k));
       if (n <= rule.lds_max) {
         *need = n;

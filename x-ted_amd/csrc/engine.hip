@@ -583,6 +583,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   pa.lchunk_tables = plan_chunk;
   pa.rule = rule;
   pa.rule_tail = ClassRule{lds_limit, 1, 1 << 30, lds_limit, 16, rule.xl_seq_rows};
+  pa.rule_tail.xl_first = rule.xl_first;
   pa.tail_level = g.tail_lv.as<uint8_t>();
   pa.force_class = force_class;
   pa.bands = size_bands;

@@ -364,8 +364,8 @@ typedef int32_t int2u __attribute__((ext_vector_type(2), aligned(4)));
 // waiting workgroup then stops waiting, and the call fails.
 // ---------------------------------------------------------------------------
 constexpr uint64_t kXLWaitTicks = 1000000000ull;  // 10 s at 100 MHz
-// A pipelined block checks its predecessor's progress every kXLEvery rows
-// (a power of two dividing 64).
+// A pipelined block checks its predecessor's progress every kXLEvery rows,
+// one window of row metadata (64 rows).
 constexpr int32_t kXLEvery = 64;
 // Tables with at most xl_seq_rows rows are swept block after block by one
 // workgroup (forms of at most 1024 columns): their words stay inside that
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // (The column-blocked form also rings its imports, and the cost form's
   // longer row body has to unroll S times: at most 8 slots for both.)
   constexpr int S0 = C >= 16 ? 2 : C >= 8 ? 4 : C >= 4 ? 8 : 16;
-  constexpr int S = kXL && S0 > 4 ? 4 : kCost && S0 > 8 ? 8 : S0;
+  constexpr int S = kXL ? (C <= 8 ? 8 : S0) : kCost && S0 > 8 ? 8 : S0;
   constexpr int PD = S - 1;  // rows of TD prefetched
   constexpr int BW = 64 * W * C;                    // columns of one block
   constexpr int KQ = kXL ? kMaxImports / 64 : 1;    // import entries per lane
@@ -752,10 +752,14 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   load_meta(0, mlab, mrf, mslot, mrid, mvp);
   if (nchunks > 1) load_meta(1, nlab, nrf, nslot, nrid, nvp);
 
-  // Column-blocked form: the upstream carry (crv) and this block's imports
+  // Column-blocked form: the upstream carries are read a window (64 rows) at
+  // a time, one window ahead, lane i holding row 64j + i's word (cwl / cwh,
+  // the next window in nwl / nwh): a read that crosses XCDs takes longer
+  // than a few rows, and loads return in order, so carries read a few rows
+  // ahead held up every later TD load behind them. This block's imports
   // (irv, wave 0 only) of row k are read into ring slot k % S when row k-PD
   // starts, with one bit each saying whether the read already carried this
-  // launch's epoch (crok, irok); when row k uses them a stale one is polled
+  // launch's epoch (irok). When row k uses a word that is stale it is polled
   // again. Reads not wanted count as valid.
   unsigned long long* xw = A.xl_words;
   int64_t cin_off = 0, cout_off = 0, exp_off = 0;
@@ -767,21 +771,25 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     cout_off = (int64_t)blk * cstride;
     exp_off = (int64_t)(nb - 1) * cstride;
   }
-  int32_t crv[kXL ? S : 1], irv[kXL ? S : 1][KQ], irid[kXL ? S : 1];
-  uint32_t crok = 0, irok[kXL ? S : 1];
+  int32_t irv[kXL ? S : 1][KQ], irid[kXL ? S : 1];
+  uint32_t irok[kXL ? S : 1];
+  uint32_t cwl = 0, cwh = 0, nwl = 0, nwh = 0;
+  auto load_cwin = [&](int32_t w0, uint32_t& lo, uint32_t& hi) {
+    if constexpr (kXL) {
+      const int32_t row = w0 + lane;
+      if (blk > 0 && row < nrows) {
+        const unsigned long long w = xl_peek(xw + cin_off + row);
+        lo = (uint32_t)w;
+        hi = (uint32_t)(w >> 32);
+      }
+    }
+  };
   // Export word of saved row rid written by block bb at depth index kx.
   auto xword = [&](int32_t rid, int32_t bb, int32_t kx) {
     return xw + exp_off + ((int64_t)rid * nb + bb) * kdsp + kx;
   };
   auto xl_issue = [&](int32_t k, int32_t rid, int slot) {  // row k of the table
     if constexpr (kXL) {
-      bool ok = true;
-      if (blk > 0 && k < nrows) {
-        const unsigned long long w = xl_peek(xw + cin_off + k);
-        crv[slot] = (int32_t)(uint32_t)w;
-        ok = (uint32_t)(w >> 32) == A.xl_epoch;
-      }
-      crok = ok ? (crok | (1u << slot)) : (crok & ~(1u << slot));
       irid[slot] = k < nrows ? rid : -1;
       uint32_t bits = 0;
 #pragma unroll
@@ -797,17 +805,14 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       irok[slot] = bits;
     }
   };
-  // Before the reads of rows k+PD .. k+PD+kXLEvery-1 are issued, the block
-  // before this one has finished every row up to the end of the line holding
-  // the last of them (its later rows' lines are not touched yet), and so have
-  // the exporters, which run ahead of it: those plain reads find their lines
-  // complete. One atomic poll per kXLEvery rows.
-  // The poll for row k's check is read 4 rows ahead (tw), so its round trip
-  // overlaps those rows.
+  // At row k = 64j, before the carries of window j+1 (rows up to 64j+127)
+  // and the imports of rows up to k+PD+63 are read, the block before this
+  // one has finished row 64j+127 (the end of a line), and so have the
+  // exporters, which run ahead of it: those reads find their lines complete.
+  // One atomic poll per kXLEvery rows, read half a window ahead (tw) so its
+  // round trip overlaps those rows.
   unsigned long long tw = 0;
-  auto xl_throttle_row = [&](int32_t k) {
-    return min((k + PD + kXLEvery - 1) | (kXLLineWords - 1), nrows - 1);
-  };
+  auto xl_throttle_row = [&](int32_t k) { return min(k + 2 * kXLEvery - 1, nrows - 1); };
   auto xl_throttle_word = [&](int32_t k) { return xw + cin_off + xl_throttle_row(k); };
   auto xl_throttle = [&](int32_t k, bool early) {
     if constexpr (kXL) {
@@ -821,6 +826,8 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   };
   if constexpr (kXL) {
     xl_throttle(0, false);
+    load_cwin(0, cwl, cwh);
+    load_cwin(kXLEvery, nwl, nwh);
 #pragma unroll
     for (int k = 0; k < PD; ++k) xl_issue(k, k < nrows ? __builtin_amdgcn_readlane(mrid, k) : -1, k);
   }
@@ -858,7 +865,12 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       load_row(max(r - PD, r0), tdq[(u + PD) % S]);
       if constexpr (kXL) {
         constexpr int32_t kM = kXLEvery - 1, kHalf = kXLEvery / 2;
-        if ((k & kM) == 0 && k > 0) xl_throttle(k, true);
+        if ((k & kM) == 0 && k > 0) {
+          cwl = nwl;
+          cwh = nwh;
+          xl_throttle(k, true);
+          load_cwin(k + kXLEvery, nwl, nwh);
+        }
         if ((k & kM) == kHalf && blk > 0 && !seq) tw = xl_load(xl_throttle_word(k + kHalf));
         const int32_t ia = i + PD;
         const int32_t rid = ia < 64 ? __builtin_amdgcn_readlane(mrid, ia)
@@ -958,8 +970,9 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       int32_t carry = kCost ? frc + (hc - hq) : (rmax - r) + 1;
       if constexpr (kXL) {
         if (blk > 0) {
-          if ((crok >> u) & 1u) {
-            carry = crv[u];
+          const uint32_t clo = __builtin_amdgcn_readlane(cwl, i);
+          if (__builtin_amdgcn_readlane(cwh, i) == A.xl_epoch) {
+            carry = (int32_t)clo;
           } else {
             ++n_stale;
             carry = xl_wait(xw + cin_off + k, 0ull, true, A.xl_epoch, A.xl_abort, ticket, k);

@@ -92,6 +92,7 @@ _SIGNATURES = {
     "xted_set_costs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _I32P, _I32P, _I32P]),
     "xted_set_tail_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_xl_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "xted_set_xl_width": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_class_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
                                           ctypes.POINTER(ctypes.c_int64),
                                           ctypes.POINTER(ctypes.c_int64),
@@ -354,6 +355,9 @@ class Context:
 
     def set_xl_rule(self, seq_rows):
         _check(lib().xted_set_xl_rule(self._h, seq_rows))
+
+    def set_xl_width(self, max_block_cols):
+        _check(lib().xted_set_xl_width(self._h, max_block_cols))
 
     def set_tail_rule(self, tail_pairs):
         _check(lib().xted_set_tail_rule(self._h, tail_pairs))
