@@ -490,7 +490,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // (The column-blocked form also rings its imports, and the cost form's
   // longer row body has to unroll S times: at most 8 slots for both.)
   constexpr int S0 = C >= 16 ? 2 : C >= 8 ? 4 : C >= 4 ? 8 : 16;
-  constexpr int S = kXL ? (C <= 8 ? 8 : S0) : kCost && S0 > 8 ? 8 : S0;
+  constexpr int S = kXL && S0 > 4 ? 4 : kCost && S0 > 8 ? 8 : S0;
   constexpr int PD = S - 1;  // rows of TD prefetched
   constexpr int BW = 64 * W * C;                    // columns of one block
   constexpr int KQ = kXL ? kMaxImports / 64 : 1;    // import entries per lane
