@@ -405,6 +405,9 @@ __device__ __forceinline__ int32_t xl_flag(int32_t* f) {
   return cur;
 }
 
+// Rows whose imports a column-blocked block reads together (one batch ahead).
+constexpr int32_t kXLImpRows = 8;
+
 // Lanes with `need` poll *p until it carries `epoch` (w is the first read).
 // abort_flag[0] is the flag; the first workgroup to give up records
 // what = (ticket, kind << 24 | row, word low, word high) in abort_flag[2..5].
@@ -500,6 +503,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // row r-1's total while a slower wave still reads row r's.
   __shared__ int32_t xtot[2][W];
   __shared__ int32_t xsh[kXL ? kMaxImports + 1 : 1];  // import flags, then the ticket
+  // (kXL) imports of depths below 64 for the rows of the current batch of
+  // kXLImpRows rows (lane = depth), and per row the lanes whose word was stale
+  __shared__ uint16_t istage[kXL ? kXLImpRows : 1][64];
+  __shared__ unsigned long long istale[kXL ? kXLImpRows : 1];
 
   int64_t tix = blockIdx.x;
   int32_t blk = 0, ticket = 0;
@@ -756,11 +763,12 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // a time, one window ahead, lane i holding row 64j + i's word (cwl / cwh,
   // the next window in nwl / nwh): a read that crosses XCDs takes longer
   // than a few rows, and loads return in order, so carries read a few rows
-  // ahead held up every later TD load behind them. This block's imports
-  // (irv, wave 0 only) of row k are read into ring slot k % S when row k-PD
-  // starts, with one bit each saying whether the read already carried this
-  // launch's epoch (irok). When row k uses a word that is stale it is polled
-  // again. Reads not wanted count as valid.
+  // ahead held up every later TD load behind them. For the same reason wave
+  // 0 reads the imports (depths below 64, lane = depth) of the saved rows of
+  // rows 8m .. 8m+7 when row 8m-8 starts (iw), and moves them into the LDS
+  // stage when row 8m starts; imports of depth 64 and above (rare) are read
+  // with atomics when their row comes. A word that is stale is polled again
+  // when its row uses it.
   unsigned long long* xw = A.xl_words;
   int64_t cin_off = 0, cout_off = 0, exp_off = 0;
   const int32_t kdsp = xl_pad(kds);
@@ -771,9 +779,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     cout_off = (int64_t)blk * cstride;
     exp_off = (int64_t)(nb - 1) * cstride;
   }
-  int32_t irv[kXL ? S : 1][KQ], irid[kXL ? S : 1];
-  uint32_t irok[kXL ? S : 1];
   uint32_t cwl = 0, cwh = 0, nwl = 0, nwh = 0;
+  unsigned long long iw[kXL ? kXLImpRows : 1];
+  bool has_imp0 = false;  // (wave 0) this block imports depths below 64
+  if constexpr (kXL) has_imp0 = __ballot(impl[0]) != 0;
   auto load_cwin = [&](int32_t w0, uint32_t& lo, uint32_t& hi) {
     if constexpr (kXL) {
       const int32_t row = w0 + lane;
@@ -788,21 +797,36 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   auto xword = [&](int32_t rid, int32_t bb, int32_t kx) {
     return xw + exp_off + ((int64_t)rid * nb + bb) * kdsp + kx;
   };
-  auto xl_issue = [&](int32_t k, int32_t rid, int slot) {  // row k of the table
+  // Rows q0 .. q0+7 (rows k0m .. k0m+63 have their metadata in mrid, the
+  // next 64 in nrid): wave 0 reads their saved rows' imports into iw.
+  auto iq_issue = [&](int32_t q0, int32_t k0m) {
     if constexpr (kXL) {
-      irid[slot] = k < nrows ? rid : -1;
-      uint32_t bits = 0;
+      if (wave == 0 && has_imp0) {
 #pragma unroll
-      for (int q = 0; q < KQ; ++q) {
-        bool okq = true;
-        if (wave == 0 && impl[q] && rid >= 0 && k < nrows) {
-          const unsigned long long w = xl_peek(xword(rid, isrc[q], lane + 64 * q));
-          irv[slot][q] = (int32_t)(uint32_t)w;
-          okq = (uint32_t)(w >> 32) == A.xl_epoch;
+        for (int r = 0; r < kXLImpRows; ++r) {
+          const int32_t k = q0 + r;
+          const int32_t ix = k - k0m;  // uniform, in [0, 128)
+          const int32_t rid = k >= nrows ? -1
+                              : ix < 64 ? __builtin_amdgcn_readlane(mrid, ix)
+                                        : __builtin_amdgcn_readlane(nrid, ix - 64);
+          iw[r] = (unsigned long long)A.xl_epoch << 32;  // nothing to read: valid
+          if (rid >= 0 && impl[0]) iw[r] = xl_peek(xword(rid, isrc[0], lane));
         }
-        bits |= okq ? (1u << q) : 0u;
       }
-      irok[slot] = bits;
+    }
+  };
+  // The batch read last time becomes the stage of the rows starting now.
+  auto iq_land = [&]() {
+    if constexpr (kXL) {
+      if (wave == 0 && has_imp0) {
+#pragma unroll
+        for (int r = 0; r < kXLImpRows; ++r) {
+          const bool stale = (uint32_t)(iw[r] >> 32) != A.xl_epoch;
+          istage[r][lane] = (uint16_t)(uint32_t)iw[r];
+          const unsigned long long m = __ballot(stale);
+          if (lane == 0) istale[r] = m;
+        }
+      }
     }
   };
   // At row k = 64j, before the carries of window j+1 (rows up to 64j+127)
@@ -828,8 +852,9 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     xl_throttle(0, false);
     load_cwin(0, cwl, cwh);
     load_cwin(kXLEvery, nwl, nwh);
-#pragma unroll
-    for (int k = 0; k < PD; ++k) xl_issue(k, k < nrows ? __builtin_amdgcn_readlane(mrid, k) : -1, k);
+    iq_issue(0, 0);
+    iq_land();
+    iq_issue(kXLImpRows, 0);
   }
 
   // F[r+1] at the column just before this wave's block (t = tb0 + w*64*C - 1)
@@ -872,10 +897,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           load_cwin(k + kXLEvery, nwl, nwh);
         }
         if ((k & kM) == kHalf && blk > 0 && !seq) tw = xl_load(xl_throttle_word(k + kHalf));
-        const int32_t ia = i + PD;
-        const int32_t rid = ia < 64 ? __builtin_amdgcn_readlane(mrid, ia)
-                                    : __builtin_amdgcn_readlane(nrid, ia - 64);
-        xl_issue(k + PD, rid, (u + PD) % S);
+        if ((k & (kXLImpRows - 1)) == 0 && k > 0) {
+          iq_land();
+          iq_issue(k + kXLImpRows, k & ~63);
+        }
       }
 
       const int32_t labr = __builtin_amdgcn_readlane(mlab, i);
@@ -1037,21 +1062,30 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
         if constexpr (kXL) {
           // Imported far values of this saved row; far values later
           // blocks import from this one.
+          const int32_t rid = __builtin_amdgcn_readlane(mrid, i);
+          if (wave == 0 && has_imp0) {
+            const int ri = k & (kXLImpRows - 1);
+            const unsigned long long sm = istale[ri];  // uniform
+            int32_t val = istage[ri][lane];
+            if (sm) {
+              ++n_stale;
+              const bool stale = (sm >> lane) & 1ull;
+              const int32_t v = xl_wait(xword(rid, isrc[0], lane), 0ull, stale, A.xl_epoch,
+                                        A.xl_abort, ticket, (1 << 24) | k);
+              if (stale) val = v;
+            }
+            if (impl[0]) wrow_ptr[wb + 1 + lane] = (uint16_t)val;
+          }
           if (wave == 0) {
 #pragma unroll
-            for (int q2 = 0; q2 < KQ; ++q2) {
-              const bool stale = impl[q2] && !((irok[u] >> q2) & 1u);
-              const bool any = __ballot(stale) != 0;  // uniform
-              n_stale += any;
-              const int32_t v =
-                  any ? xl_wait(xword(max(irid[u], 0), isrc[q2], lane + 64 * q2), 0ull, stale,
-                                A.xl_epoch, A.xl_abort, ticket, (1 << 24) | k)
-                      : 0;
-              const int32_t val = stale ? v : irv[u][q2];
-              if (impl[q2]) wrow_ptr[wb + 1 + lane + 64 * q2] = (uint16_t)val;
+            for (int q2 = 1; q2 < KQ; ++q2) {  // depths 64 and above: read when needed
+              if (__ballot(impl[q2]) != 0) {  // uniform
+                const int32_t v = xl_wait(xword(rid, isrc[q2], lane + 64 * q2), 0ull, impl[q2],
+                                          A.xl_epoch, A.xl_abort, ticket, (1 << 24) | k);
+                if (impl[q2]) wrow_ptr[wb + 1 + lane + 64 * q2] = (uint16_t)v;
+              }
             }
           }
-          const int32_t rid = __builtin_amdgcn_readlane(mrid, i);
 #pragma unroll
           for (int j = 0; j < C; ++j) {
             if (xinf[j] >= 0) xl_store(xword(rid, blk, xinf[j]), A.xl_epoch, h[j] - term(j), seq);
