@@ -12,8 +12,10 @@ The same JSON line also reports:
   * the plain per-pair drop-in path (xted_ted: upload, plan, DP, full-table
     read-back per pair) on a sample of the same pairs;
   * GCUPS (interior forest cells / s);
-  * the dominant kernel's roofline: 4 bytes of TD per cell against 8 TB/s
-    HBM, timed with events around each of its launches;
+  * the dominant kernel's roofline: 4 bytes of TD per cell (one int32 entry
+    of the tree-distance table) against 8 TB/s HBM, timed with events around
+    each of its launches; the arena stores 16-bit entries, so the bytes it
+    moves are half that (frac_at_arena_bytes beside it);
   * the reference's own X-TED_CPU (compiled from its sources into
     oracle/_ref/) on a bounded sample of the same pairs, on this host;
   * all-pairs mode: the distance matrix of the first 128 trees (8128 pairs);
@@ -42,7 +44,10 @@ sys.path.insert(0, os.path.join(ROOT, "x-ted_amd"))
 import xted  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per interior cell: one int32 TD entry read (forest cells)
+# or written (tree cells). The arena holds each entry in 16 bits.
 BYTES_PER_CELL = 4
+ARENA_BYTES_PER_CELL = 2
 # Seed streams of the synthetic workloads (xted.derive_seed(stream, tree index)).
 SEED_BATCH = 3
 SEED_SERVE = 5
@@ -336,13 +341,16 @@ def traffic_from_profile(kernel):
 
 def dominant_roofline(prof, suffix=""):
     """Roofline of the class with the most device time in a profiled step
-    (4 bytes of TD per interior cell over its event-timed launches) and the
+    (4 bytes of TD per interior cell over its event-timed launches, and the
+    fraction at the arena's 2 bytes per cell) and the
     per-class table. suffix names the kernel variant in the profiles."""
     cls, (launches, tables, cells, ms) = max(prof.items(), key=lambda kv: kv[1][3])
     name = xted.class_name(cls) + suffix
     achieved = BYTES_PER_CELL * cells / (ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "frac_at_arena_bytes": round(achieved * ARENA_BYTES_PER_CELL / BYTES_PER_CELL
+                                             / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profile(name),
                 "kernel": name, "launches": launches,
                 "avg_launch_ms": round(ms / launches, 4),

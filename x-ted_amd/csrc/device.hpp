@@ -13,7 +13,8 @@
 //   spre           [node_off + k], k in [0, K]: prefix sums of ssize
 //   kdoff          [node_off + d], d in [0, kdmax+1]
 // A pair owns a tree-distance table TD (m x n, row-major, preorder) and its
-// transpose TDT (n x m); both live in one int32 arena at td_off / tdt_off.
+// transpose TDT (n x m); both live in one arena of 16-bit entries (td_t) at
+// td_off / tdt_off.
 #pragma once
 #include <cstdint>
 
@@ -35,10 +36,18 @@ struct TreeMeta {
   int32_t pad;
 };
 
+// One entry of a tree-distance table in the arena. Every entry is at most
+// the cost of deleting all of x and inserting all of y: with unit costs
+// n_x + n_y <= 65534 (pool trees have at most 32767 nodes), under a cost
+// model at most 65535 (checked per pair on the host). So 16 bits hold every
+// value exactly, and each forest cell reads 2 bytes of HBM instead of 4;
+// the kernels compute in int32 and the tables handed to a caller are int32.
+using td_t = uint16_t;
+
 struct PairMeta {
   int32_t tx, ty;   // tree ids (rows = x, columns = y)
-  int64_t td_off;   // TD  offset (m*n ints)
-  int64_t tdt_off;  // TDT offset (n*m ints)
+  int64_t td_off;   // TD  offset (m*n entries)
+  int64_t tdt_off;  // TDT offset (n*m entries)
 };
 
 // What a forest kernel needs to start one table, written by the plan next to

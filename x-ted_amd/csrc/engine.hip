@@ -510,7 +510,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
     st.tables += (int64_t)a.K * b.K;
     st.cells += info_[x].ksum * info_[y].ksum;
   }
-  g.td_bytes = (int64_t)sizeof(int32_t) * td_total;
+  g.td_bytes = (int64_t)sizeof(td_t) * td_total;
   const int nb = nlevels * kNumClasses * kBands;
   g.loff.assign(npairs, 0);
   if (level_align == 1 || level_align == 2) {
@@ -740,7 +740,7 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
       DPArgs da;
       da.pool = view();
       da.pairs = g.pairs.as<PairMeta>();
-      da.td = g.td.as<int32_t>();
+      da.td = g.td.as<td_t>();
       da.tasks = g.tasks.as<uint64_t>() + w.off;
       da.descs = g.descs.as<TaskDesc>() + w.off;
       da.ntasks = (int64_t)w.cnt;
@@ -806,7 +806,7 @@ int Context::finish_async(Group& g, int lane) {
   hipStream_t ls = lane_stream(lane);
   ST_TRY(reserve_pinned(g, g.npairs));
   if (!g.done_ev) HIP_TRY(hipEventCreateWithFlags(&g.done_ev, hipEventDisableTiming));
-  launch_gather_dist(g.pairs.as<PairMeta>(), g.td.as<int32_t>(), g.npairs, g.dist.as<int32_t>(),
+  launch_gather_dist(g.pairs.as<PairMeta>(), g.td.as<td_t>(), g.npairs, g.dist.as<int32_t>(),
                      ls);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(g.h_dist_pin, g.dist.p, sizeof(int32_t) * g.npairs,
@@ -864,7 +864,7 @@ LeafArgs Context::leaf_args(const Group& g) const {
   la.pairs = g.pairs.as<PairMeta>();
   la.tile_start = g.lines_d.as<int64_t>();
   la.npairs = g.npairs;
-  la.td = g.td.as<int32_t>();
+  la.td = g.td.as<td_t>();
   la.del = cdel_.as<int32_t>();
   la.ins = cins_.as<int32_t>();
   la.ren = cren_.as<int32_t>();
@@ -877,7 +877,7 @@ int64_t Context::pair_device_bytes(int32_t x, int32_t y) const {
   const TreeMeta &a = meta_[x], &b = meta_[y];
   const bool lf = leaf_form(x, y);
   const int64_t ka = lf ? a.K - a.K1 : a.K, kb = lf ? b.K - b.K1 : b.K;
-  return 8 * (int64_t)a.n * b.n + 3 * kTdPad * (int64_t)sizeof(int32_t) +
+  return (2 * (int64_t)a.n * b.n + 3 * kTdPad) * (int64_t)sizeof(td_t) +
          (int64_t)(sizeof(uint64_t) + sizeof(TaskDesc)) * ka * kb;
 }
 
@@ -994,7 +994,7 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
     hipStream_t s = lane_stream(ng > 1 ? k : -1);
     gd[k].assign(g.npairs, 0);
     aborted[k].fill(0);
-    launch_gather_dist(g.pairs.as<PairMeta>(), g.td.as<int32_t>(), g.npairs, g.dist.as<int32_t>(),
+    launch_gather_dist(g.pairs.as<PairMeta>(), g.td.as<td_t>(), g.npairs, g.dist.as<int32_t>(),
                        s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(gd[k].data(), g.dist.p, sizeof(int32_t) * g.npairs,
@@ -1003,12 +1003,20 @@ int Context::run_batch(int32_t npairs, const int32_t* xs, const int32_t* ys, int
       HIP_TRY(hipMemcpyAsync(aborted[k].data(), g.abort.p, sizeof(int32_t) * kAbortInts,
                              hipMemcpyDeviceToHost, s));
     if (td_out) {
+      // Each requested table is widened to int32 in a staging buffer, then
+      // copied out (in stream order, so the next widening waits for it).
+      int64_t most = 0;
+      for (int32_t j = 0; j < g.npairs; ++j)
+        if (td_out[idx[k][j]])
+          most = std::max(most, (int64_t)meta_[g.xs[j]].n * meta_[g.ys[j]].n);
+      if (most > 0) ST_TRY(g.tdw.grow(sizeof(int32_t) * (size_t)most, false, s));
       for (int32_t j = 0; j < g.npairs; ++j) {
         int32_t* out = td_out[idx[k][j]];
         if (!out) continue;
         const int64_t mn = (int64_t)meta_[g.xs[j]].n * meta_[g.ys[j]].n;
-        HIP_TRY(hipMemcpyAsync(out, g.td.as<int32_t>() + g.pm[j].td_off, sizeof(int32_t) * mn,
-                               hipMemcpyDeviceToHost, s));
+        launch_widen_td(g.td.as<td_t>() + g.pm[j].td_off, g.tdw.as<int32_t>(), mn, s);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, g.tdw.p, sizeof(int32_t) * mn, hipMemcpyDeviceToHost, s));
       }
     }
   }
@@ -1075,7 +1083,8 @@ int64_t Context::noalloc_bytes(int32_t npairs, const int32_t* xs, const int32_t*
   const int64_t nb = (int64_t)nlevels * kNumClasses * kBands;
   planned = std::max<int64_t>(planned, 1);
   return r(sizeof(PairMeta) * npairs) + r(nlevels) + 2 * r(8 * (int64_t)(npairs + 1)) +
-         3 * r(8 * nb) + 2 * r(4 * nb) + r(32) + r(8 * planned) + r(64 * planned) + r(4 * td) +
+         3 * r(8 * nb) + 2 * r(4 * nb) + r(32) + r(8 * planned) + r(64 * planned) +
+         r((int64_t)sizeof(td_t) * td) +
          r(4 * (int64_t)npairs) + (level_align ? r(4 * (int64_t)npairs) : 0) + r(npairs);
 }
 
@@ -1125,7 +1134,7 @@ int Context::run_batch_noalloc(int32_t npairs, const int32_t* xs, const int32_t*
   }
   hipStream_t s = lane_stream(-1);
   int32_t rec[kAbortInts] = {};
-  launch_gather_dist(na_.pairs.as<PairMeta>(), na_.td.as<int32_t>(), npairs,
+  launch_gather_dist(na_.pairs.as<PairMeta>(), na_.td.as<td_t>(), npairs,
                      na_.dist.as<int32_t>(), s);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(dist, na_.dist.p, sizeof(int32_t) * npairs, hipMemcpyDeviceToHost, s));

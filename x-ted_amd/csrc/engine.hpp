@@ -93,6 +93,7 @@ struct Group {
   std::vector<uint8_t> h_tail;
   DevBuf pairs, blk, counts, cells, cursor, maxlds, tickets, tasks, descs, td, dist, lines_d,
       tail_lv, xl_off, xl_total, loff_d, leaf_d;
+  DevBuf tdw;  // requested tables widened to int32 on their way out
   DevBuf abort, xl_words;
   // finish_async: distances then the abort flag land in pinned memory
   int32_t* h_dist_pin = nullptr;
@@ -116,7 +117,7 @@ struct Group {
   void for_each_buf(F f) {
     for (DevBuf* b : {&pairs, &blk, &counts, &cells, &cursor, &maxlds, &tickets, &abort, &tasks,
                       &descs, &td, &dist, &lines_d, &tail_lv, &xl_off, &xl_total, &xl_words,
-                      &loff_d, &leaf_d})
+                      &loff_d, &leaf_d, &tdw})
       f(*b);
   }
 };

@@ -122,8 +122,8 @@ __global__ __launch_bounds__(256) void leaf_kernel(LeafArgs A, int64_t ntiles, i
   const int32_t* __restrict__ kdnT = A.pool.kdn + T.node_off;
   // of_x: TD[v][c] = td[td_off + v*ny + c], TD^T[c][v] = td[tdt_off + c*nx + v];
   // else: TD^T[v][c] = td[tdt_off + v*nx + c], TD[c][v] = td[td_off + c*ny + v].
-  int32_t* const rows = A.td + (of_x ? pm.td_off : pm.tdt_off);  // [v][c], stride T.n
-  int32_t* const cols = A.td + (of_x ? pm.tdt_off : pm.td_off);  // [c][v], stride R.n
+  td_t* const rows = A.td + (of_x ? pm.td_off : pm.tdt_off);  // [v][c], stride T.n
+  td_t* const cols = A.td + (of_x ? pm.tdt_off : pm.td_off);  // [c][v], stride R.n
   const bool col_ok = vl < R.n;
   __syncthreads();
   constexpr int kPer = kLeafTile / 4;  // lines (and positions) per wave
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void leaf_kernel(LeafArgs A, int64_t ntiles, i
       const int32_t nxt = min(carry[m], incl);
       carry[m] = min(carry[m], __builtin_amdgcn_readlane(incl, 63));
       const int32_t d = size - (int32_t)(nxt <= oc);
-      if (real && c >= 0) rows[(int64_t)(v0 + off) * T.n + c] = d;
+      if (real && c >= 0) rows[(int64_t)(v0 + off) * T.n + c] = (td_t)d;
       tile[off][lane] = d;
     }
     const unsigned long long skip = __ballot(c < 0 || c_line);
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void leaf_kernel(LeafArgs A, int64_t ntiles, i
     for (int m = 0; m < kPer; ++m) {
       const int p = wave + 4 * m;
       if (!((skip >> p) & 1ull) && col_ok)
-        cols[(int64_t)(top - p) * R.n + vl] = tile[lane][p];
+        cols[(int64_t)(top - p) * R.n + vl] = (td_t)tile[lane][p];
     }
     __syncthreads();
   }
@@ -236,8 +236,8 @@ __global__ __launch_bounds__(64) void leaf_cost_kernel(LeafArgs A, int64_t ntile
   // ren(v, k) = ren[labv * a + lab k] for v of x, ren[lab k * a + labv] for v of y.
   const int32_t* const renv = A.ren + (of_x ? (int64_t)labv * A.alphabet : labv);
   const int32_t rstride = of_x ? 1 : A.alphabet;
-  int32_t* const rows = A.td + (of_x ? pm.td_off : pm.tdt_off);  // [v][c], stride T.n
-  int32_t* const cols = A.td + (of_x ? pm.tdt_off : pm.td_off);  // [c][v], stride R.n
+  td_t* const rows = A.td + (of_x ? pm.td_off : pm.tdt_off);  // [v][c], stride T.n
+  td_t* const cols = A.td + (of_x ? pm.tdt_off : pm.td_off);  // [c][v], stride R.n
   const bool col_ok = vl < R.n;
   // Stack entries [0, sp-1) in LDS, entry sp-1 in tos.
   int32_t sp = 0, rprev = 0, tos = 0;
@@ -295,13 +295,13 @@ __global__ __launch_bounds__(64) void leaf_cost_kernel(LeafArgs A, int64_t ntile
     while (lm) {
       const int l = __ffsll((long long)lm) - 1;
       lm &= lm - 1;
-      if (pos_ok) rows[(int64_t)(v0 + l) * T.n + (top - lane)] = (int32_t)tile[lane][l];
+      if (pos_ok) rows[(int64_t)(v0 + l) * T.n + (top - lane)] = tile[lane][l];
     }
     // Column layout: position top - i across the tile's 64 nodes.
     const unsigned long long skip = __ballot(!pos_ok || (flags & 8));
     for (int i = 0; i < cnt; ++i) {
       if (!((skip >> i) & 1ull) && col_ok)
-        cols[(int64_t)(top - i) * R.n + vl] = (int32_t)tile[i][lane];
+        cols[(int64_t)(top - i) * R.n + vl] = tile[i][lane];
     }
     __syncthreads();
   }
@@ -335,8 +335,12 @@ void launch_leaf(const LeafArgs& a, int64_t ntiles, hipStream_t s) {
 // reads with one coalesced E-int load per lane, and the row scan is G
 // chained wave scans over groups.
 // ---------------------------------------------------------------------------
-typedef int32_t int4u __attribute__((ext_vector_type(4), aligned(4)));
-typedef int32_t int2u __attribute__((ext_vector_type(2), aligned(4)));
+// Words of a TD row group: 2, 4 or 8 entries of 16 bits (td_t), two to a
+// 32-bit word, at any even byte address (the target's unaligned-access mode
+// lets one dwordx2 / dwordx4 load read them).
+typedef uint32_t u32x4a2 __attribute__((ext_vector_type(4), aligned(2)));
+typedef uint32_t u32x2a2 __attribute__((ext_vector_type(2), aligned(2)));
+typedef uint32_t u32x1a2 __attribute__((aligned(2)));
 
 // ---------------------------------------------------------------------------
 // Column-blocked form (kXL). A table too wide or too LDS-hungry for one
@@ -503,10 +507,11 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // row r-1's total while a slower wave still reads row r's.
   __shared__ int32_t xtot[2][W];
   __shared__ int32_t xsh[kXL ? kMaxImports + 1 : 1];  // import flags, then the ticket
-  // (kXL) imports of depths below 64 for the rows of the current batch of
-  // kXLImpRows rows (lane = depth), and per row the lanes whose word was stale
-  __shared__ uint16_t istage[kXL ? kXLImpRows : 1][64];
-  __shared__ unsigned long long istale[kXL ? kXLImpRows : 1];
+  // (kXL) imports of the rows of the current batch of kXLImpRows rows, by
+  // depth, and per row and quarter of depths (64 each) the lanes whose word
+  // was stale. Wave w stages quarters w, w + W, ... and is their only reader.
+  __shared__ uint16_t istage[kXL ? kXLImpRows : 1][kXL ? kMaxImports : 1];
+  __shared__ unsigned long long istale[kXL ? kXLImpRows : 1][KQ];
 
   int64_t tix = blockIdx.x;
   int32_t blk = 0, ticket = 0;
@@ -534,8 +539,8 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   const int32_t* __restrict__ kdepC = A.pool.kdep + offC;
   const int32_t* __restrict__ rtopC = A.pool.rtop + offC;
   const int32_t r0 = dsc.r0, c0 = dsc.c0;
-  int32_t* TDo = A.td + dsc.tdo;  // [r][c]
-  int32_t* TDt = A.td + dsc.tdt;  // [c][r]
+  td_t* TDo = A.td + dsc.tdo;  // [r][c]
+  td_t* TDt = A.td + dsc.tdt;  // [c][r]
   const int64_t ldo = dsc.ldo;
   const int64_t ldt = dsc.ldt;
 
@@ -627,7 +632,11 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // scan order, so whatever they hold never reaches a real column; only
   // their stores are masked.
   bool tcol[C];
-  int32_t lb[C], foff[C], woff[C], Fpm[C], tdq[S][C];
+  // tdq: the TD ring, as loaded: two 16-bit entries per word (one per word
+  // when E = 1), unpacked where a row uses them (tdv).
+  constexpr int NW = E == 1 ? C : C / 2;
+  int32_t lb[C], foff[C], woff[C], Fpm[C];
+  uint32_t tdq[S][NW];
   int32_t negt[kCost ? C : 1];
   int32_t xinf[kXL ? C : 1];
   auto term = [&](int j) -> int32_t {
@@ -680,54 +689,63 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       }
     }
   }
-  // Which import entries this block uses (lane l holds depths l + 64q), and
-  // the block each comes from.
-  bool impl[KQ];
-  int32_t isrc[KQ];
+  // Which import entries this block uses and the block each comes from:
+  // this wave's quarters of depths q = wave + W*m, lane l holding depth
+  // l + 64q; hasq[m] when any lane of quarter m imports (uniform).
+  constexpr int QW = (KQ + W - 1) / W;
+  bool impl[QW], hasq[QW];
+  int32_t isrc[QW];
   if constexpr (kXL) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < KQ; ++q) {
-      impl[q] = xsh[lane + 64 * q] != 0;
-      isrc[q] = max(xsh[lane + 64 * q] - 1, 0);
+    for (int m = 0; m < QW; ++m) {
+      const int q = wave + W * m;
+      const int32_t v = q < KQ ? xsh[lane + 64 * q] : 0;
+      impl[m] = v != 0;
+      isrc[m] = max(v - 1, 0);
+      hasq[m] = __ballot(impl[m]) != 0;
     }
   }
 
-  // TD row rr of this lane's columns into out[]. Loads are unconditional, so
-  // the compiler never waits for them inside a branch: a group with no valid
-  // column reads the row's first E ints instead, a partly valid group reads
-  // up to E-1 ints before column c0 (the arena keeps kTdPad ints of padding
-  // before and after every table, so both stay inside the allocation).
-  auto load_row = [&](int32_t rr, int32_t* out) {
-    const int32_t* row = TDo + (int64_t)rr * ldo;
+  // TD row rr of this lane's columns into out[]: entry k of a group (column
+  // off + k) is half k & 1 of word k / 2 of the group. Loads are
+  // unconditional, so the compiler never waits for them inside a branch: a
+  // group with no valid column reads E entries from column c0 instead, a
+  // partly valid group reads up to E-1 entries before column c0 (the arena
+  // keeps kTdPad entries of padding before and after every table, so both
+  // stay inside the allocation).
+  auto load_row = [&](int32_t rr, uint32_t* out) {
+    const td_t* row = TDo + (int64_t)rr * ldo;
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int32_t tb = TCOL(g, 0);
       const int32_t off = tb < ncols ? cmax - E - tb : c0;
       if constexpr (E == 8) {
-        const int4u lo = *reinterpret_cast<const int4u*>(row + off);
-        const int4u hi = *reinterpret_cast<const int4u*>(row + off + 4);
-        out[g * 8 + 0] = hi.w;
-        out[g * 8 + 1] = hi.z;
-        out[g * 8 + 2] = hi.y;
-        out[g * 8 + 3] = hi.x;
-        out[g * 8 + 4] = lo.w;
-        out[g * 8 + 5] = lo.z;
-        out[g * 8 + 6] = lo.y;
-        out[g * 8 + 7] = lo.x;
+        const u32x4a2 v = *reinterpret_cast<const u32x4a2*>(row + off);
+        out[g * 4 + 0] = v.x;
+        out[g * 4 + 1] = v.y;
+        out[g * 4 + 2] = v.z;
+        out[g * 4 + 3] = v.w;
       } else if constexpr (E == 4) {
-        const int4u v = *reinterpret_cast<const int4u*>(row + off);
-        out[g * 4 + 0] = v.w;
-        out[g * 4 + 1] = v.z;
-        out[g * 4 + 2] = v.y;
-        out[g * 4 + 3] = v.x;
+        const u32x2a2 v = *reinterpret_cast<const u32x2a2*>(row + off);
+        out[g * 2 + 0] = v.x;
+        out[g * 2 + 1] = v.y;
       } else if constexpr (E == 2) {
-        const int2u v = *reinterpret_cast<const int2u*>(row + off);
-        out[g * 2 + 0] = v.y;
-        out[g * 2 + 1] = v.x;
+        out[g] = *reinterpret_cast<const u32x1a2*>(row + off);
       } else {
         out[g] = row[off];
       }
+    }
+  };
+  // Element j of ring slot u as int32: column t = TCOL(j / E, j % E) is
+  // entry E-1-(j % E) of its group.
+  auto tdv = [&](int u, int j) -> int32_t {
+    if constexpr (E == 1) {
+      return (int32_t)tdq[u][j];
+    } else {
+      const int k = E - 1 - j % E;
+      const uint32_t w = tdq[u][(j / E) * (E / 2) + k / 2];
+      return (k & 1) ? (int32_t)(w >> 16) : (int32_t)(w & 0xFFFFu);
     }
   };
 #pragma unroll
@@ -763,12 +781,11 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // a time, one window ahead, lane i holding row 64j + i's word (cwl / cwh,
   // the next window in nwl / nwh): a read that crosses XCDs takes longer
   // than a few rows, and loads return in order, so carries read a few rows
-  // ahead held up every later TD load behind them. For the same reason wave
-  // 0 reads the imports (depths below 64, lane = depth) of the saved rows of
-  // rows 8m .. 8m+7 when row 8m-8 starts (iw), and moves them into the LDS
-  // stage when row 8m starts; imports of depth 64 and above (rare) are read
-  // with atomics when their row comes. A word that is stale is polled again
-  // when its row uses it.
+  // ahead held up every later TD load behind them. For the same reason the
+  // imports of the saved rows of rows 8m .. 8m+7 are read when row 8m-8
+  // starts (iw: wave w reads the quarters of depths q = w, w + W, ..., lane
+  // l depth l + 64q) and moved into the LDS stage when row 8m starts; a
+  // word that is stale is polled again when its row uses it.
   unsigned long long* xw = A.xl_words;
   int64_t cin_off = 0, cout_off = 0, exp_off = 0;
   const int32_t kdsp = xl_pad(kds);
@@ -780,9 +797,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     exp_off = (int64_t)(nb - 1) * cstride;
   }
   uint32_t cwl = 0, cwh = 0, nwl = 0, nwh = 0;
-  unsigned long long iw[kXL ? kXLImpRows : 1];
-  bool has_imp0 = false;  // (wave 0) this block imports depths below 64
-  if constexpr (kXL) has_imp0 = __ballot(impl[0]) != 0;
+  unsigned long long iw[QW][kXL ? kXLImpRows : 1];
   auto load_cwin = [&](int32_t w0, uint32_t& lo, uint32_t& hi) {
     if constexpr (kXL) {
       const int32_t row = w0 + lane;
@@ -798,10 +813,13 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     return xw + exp_off + ((int64_t)rid * nb + bb) * kdsp + kx;
   };
   // Rows q0 .. q0+7 (rows k0m .. k0m+63 have their metadata in mrid, the
-  // next 64 in nrid): wave 0 reads their saved rows' imports into iw.
+  // next 64 in nrid): each wave reads its quarters' imports of their saved
+  // rows into iw.
   auto iq_issue = [&](int32_t q0, int32_t k0m) {
     if constexpr (kXL) {
-      if (wave == 0 && has_imp0) {
+#pragma unroll
+      for (int m = 0; m < QW; ++m) {
+        if (!hasq[m]) continue;  // uniform
 #pragma unroll
         for (int r = 0; r < kXLImpRows; ++r) {
           const int32_t k = q0 + r;
@@ -809,8 +827,9 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           const int32_t rid = k >= nrows ? -1
                               : ix < 64 ? __builtin_amdgcn_readlane(mrid, ix)
                                         : __builtin_amdgcn_readlane(nrid, ix - 64);
-          iw[r] = (unsigned long long)A.xl_epoch << 32;  // nothing to read: valid
-          if (rid >= 0 && impl[0]) iw[r] = xl_peek(xword(rid, isrc[0], lane));
+          iw[m][r] = (unsigned long long)A.xl_epoch << 32;  // nothing to read: valid
+          if (rid >= 0 && impl[m])
+            iw[m][r] = xl_peek(xword(rid, isrc[m], lane + 64 * (wave + W * m)));
         }
       }
     }
@@ -818,13 +837,16 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // The batch read last time becomes the stage of the rows starting now.
   auto iq_land = [&]() {
     if constexpr (kXL) {
-      if (wave == 0 && has_imp0) {
+#pragma unroll
+      for (int m = 0; m < QW; ++m) {
+        if (!hasq[m]) continue;  // uniform
+        const int q = wave + W * m;
 #pragma unroll
         for (int r = 0; r < kXLImpRows; ++r) {
-          const bool stale = (uint32_t)(iw[r] >> 32) != A.xl_epoch;
-          istage[r][lane] = (uint16_t)(uint32_t)iw[r];
-          const unsigned long long m = __ballot(stale);
-          if (lane == 0) istale[r] = m;
+          const bool stale = (uint32_t)(iw[m][r] >> 32) != A.xl_epoch;
+          istage[r][lane + 64 * q] = (uint16_t)(uint32_t)iw[m][r];
+          const unsigned long long sm = __ballot(stale);
+          if (lane == 0) istale[r][q] = sm;
         }
       }
     }
@@ -970,11 +992,11 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
                 rename = (int32_t)(labr != lb[j]);
               }
               const int32_t diag = fprm + rename;
-              const int32_t far = lb[j] + tdq[u][j] + term(j);
+              const int32_t far = lb[j] + tdv(u, j) + term(j);
               // Blended rather than branched per element.
               dm = tcol[j] ? diag : far;
             } else {
-              dm = sv[j] + tdq[u][j] + term(j);
+              dm = sv[j] + tdv(u, j) + term(j);
             }
             h[j] = min(dm, Fpm[j]);
             if (e) h[j] = min(h[j], h[j - 1]);
@@ -999,7 +1021,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           if (__builtin_amdgcn_readlane(cwh, i) == A.xl_epoch) {
             carry = (int32_t)clo;
           } else {
-            ++n_stale;
+            n_stale += wave == 0;
             carry = xl_wait(xw + cin_off + k, 0ull, true, A.xl_epoch, A.xl_abort, ticket, k);
           }
         }
@@ -1041,16 +1063,16 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
         // keeping two 64-bit addresses per element live across the loop.
         int32_t cb = cmax - 1 - tlane;
         asm volatile("" : "+v"(cb));
-        int32_t* const rowp = TDo + (int64_t)r * ldo;
-        int32_t* const colp = TDt + r;
+        td_t* const rowp = TDo + (int64_t)r * ldo;
+        td_t* const colp = TDt + r;
         const uint32_t ldt32 = (uint32_t)ldt;
 #pragma unroll
         for (int j = 0; j < C; ++j) {
           const int32_t c = cb - ((j / E) * 64 * E + j % E);
           if (c >= c0 && tcol[j]) {
             // c * ldt < 2^30: trees have at most 32767 nodes
-            rowp[c] = h[j] - term(j);
-            colp[(uint32_t)c * ldt32] = h[j] - term(j);
+            rowp[c] = (td_t)(h[j] - term(j));
+            colp[(uint32_t)c * ldt32] = (td_t)(h[j] - term(j));
           }
         }
       }
@@ -1063,28 +1085,21 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           // Imported far values of this saved row; far values later
           // blocks import from this one.
           const int32_t rid = __builtin_amdgcn_readlane(mrid, i);
-          if (wave == 0 && has_imp0) {
-            const int ri = k & (kXLImpRows - 1);
-            const unsigned long long sm = istale[ri];  // uniform
-            int32_t val = istage[ri][lane];
+          const int ri = k & (kXLImpRows - 1);
+#pragma unroll
+          for (int m = 0; m < QW; ++m) {
+            if (!hasq[m]) continue;  // uniform
+            const int q = wave + W * m;
+            const unsigned long long sm = istale[ri][q];  // uniform
+            int32_t val = istage[ri][lane + 64 * q];
             if (sm) {
               ++n_stale;
               const bool stale = (sm >> lane) & 1ull;
-              const int32_t v = xl_wait(xword(rid, isrc[0], lane), 0ull, stale, A.xl_epoch,
-                                        A.xl_abort, ticket, (1 << 24) | k);
+              const int32_t v = xl_wait(xword(rid, isrc[m], lane + 64 * q), 0ull, stale,
+                                        A.xl_epoch, A.xl_abort, ticket, (1 << 24) | k);
               if (stale) val = v;
             }
-            if (impl[0]) wrow_ptr[wb + 1 + lane] = (uint16_t)val;
-          }
-          if (wave == 0) {
-#pragma unroll
-            for (int q2 = 1; q2 < KQ; ++q2) {  // depths 64 and above: read when needed
-              if (__ballot(impl[q2]) != 0) {  // uniform
-                const int32_t v = xl_wait(xword(rid, isrc[q2], lane + 64 * q2), 0ull, impl[q2],
-                                          A.xl_epoch, A.xl_abort, ticket, (1 << 24) | k);
-                if (impl[q2]) wrow_ptr[wb + 1 + lane + 64 * q2] = (uint16_t)v;
-              }
-            }
+            if (impl[m]) wrow_ptr[wb + 1 + lane + 64 * q] = (uint16_t)val;
           }
 #pragma unroll
           for (int j = 0; j < C; ++j) {
@@ -1100,10 +1115,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   }
   } while (kSeqForm && seq && ++blk < nb);  // blocks of a table swept by one workgroup
   if constexpr (kXL) {
-    if (threadIdx.x == 0 && (n_stale | n_wait)) {
-      atomicAdd(A.xl_abort + 8, n_stale);
-      atomicAdd(A.xl_abort + 9, n_wait);
-    }
+    // Stale reads: carries counted by wave 0, imports by the wave that
+    // stages them; pipeline waits by wave 0 (every wave waits alike).
+    if ((threadIdx.x & 63) == 0 && n_stale) atomicAdd(A.xl_abort + 8, n_stale);
+    if (threadIdx.x == 0 && n_wait) atomicAdd(A.xl_abort + 9, n_wait);
   }
 #undef TCOL
 }
@@ -1155,8 +1170,8 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
   const int32_t* __restrict__ labC = A.pool.label + offC;
   const int32_t* __restrict__ lrankC = A.pool.lrank + offC;
   const int32_t r0 = dsc.r0, c0 = dsc.c0;
-  int32_t* TDo = A.td + dsc.tdo;  // [r][c]
-  int32_t* TDt = A.td + dsc.tdt;  // [c][r]
+  td_t* TDo = A.td + dsc.tdo;  // [r][c]
+  td_t* TDt = A.td + dsc.tdt;  // [c][r]
   const int64_t ldo = dsc.ldo;
   const int64_t ldt = dsc.ldt;
   const int32_t rmax = dsc.rmax, cmax = dsc.cmax;
@@ -1265,8 +1280,8 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
       Fpm = h + vnext;
       left_prev = frc;
       if (tree_row && tree_col && col_ok) {
-        TDo[(int64_t)r * ldo + c] = h - negt;
-        TDt[(int64_t)c * ldt + r] = h - negt;
+        TDo[(int64_t)r * ldo + c] = (td_t)(h - negt);
+        TDt[(int64_t)c * ldt + r] = (td_t)(h - negt);
       }
       if (sl < 0) {
         uint16_t* wrow = slots + ((sl >> 16) & 0x7FFF) * rowlen;

@@ -84,8 +84,11 @@ inline int64_t plan_blocks(int32_t Kx, int32_t Ky, int32_t chunk) {
   return (Ky + lper - 1) / lper;
 }
 void launch_plan(const PlanArgs& a, int64_t nblocks, hipStream_t s);
-void launch_gather_dist(const PairMeta* pairs, const int32_t* td, int32_t npairs, int32_t* dist,
+void launch_gather_dist(const PairMeta* pairs, const td_t* td, int32_t npairs, int32_t* dist,
                         hipStream_t s);
+// dst[i] = src[i] for i < n: a table of the arena widened to the int32 a
+// caller receives.
+void launch_widen_td(const td_t* src, int32_t* dst, int64_t n, hipStream_t s);
 
 // Tables with a leaf keyroot, in closed form. For a single node v and a
 // subtree T (unit costs), TED({v}, T) = |T| - [label(v) occurs in T]: map v
@@ -104,7 +107,7 @@ struct LeafArgs {
   // npairs+1: prefix over pairs of ceil(n_x / 64) + ceil(n_y / 64)
   const int64_t* tile_start;
   int32_t npairs;
-  int32_t* td;
+  td_t* td;
   // Cost model (costs != 0): per-label delete / insert costs and the rename
   // matrix ren[a * alphabet + b]; the pool's pdel / pins hold their prefix
   // sums over each tree's nodes.
@@ -132,15 +135,16 @@ void launch_leaf(const LeafArgs& a, int64_t ntiles, hipStream_t s);
 // Ints of a group's column-blocked record (DPArgs::xl_abort).
 constexpr int kAbortInts = 16;
 
-// Ints of padding the TD arena keeps before and after every table, so the
-// row-scan kernels can issue whole-group loads without bounds branches.
-constexpr int kTdPad = 8;
+// Entries of padding the TD arena keeps before and after every table, so
+// the row-scan kernels can issue whole-group loads without bounds branches
+// (a group of 8 entries starts at most 7 before column c0 or at c0 itself).
+constexpr int kTdPad = 16;
 
 // Forest DP.
 struct DPArgs {
   PoolView pool;
   const PairMeta* pairs;
-  int32_t* td;            // arena holding every TD / TDT
+  td_t* td;               // arena holding every TD / TDT
   const uint64_t* tasks;  // this launch's tables
   const TaskDesc* descs;  // and their descriptors
   int64_t ntasks;

@@ -11,6 +11,7 @@
 // workgroup: each distinct bucket of a workgroup costs one global atomic, and
 // a wave's tables of one bucket land contiguously.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 
 #include "kernels.hpp"
 
@@ -380,17 +381,30 @@ void launch_plan(const PlanArgs& a, int64_t nblocks, hipStream_t s) {
 }
 
 // Distances TD[0][0] of every pair into dist[].
-__global__ void gather_dist_kernel(const PairMeta* pairs, const int32_t* td, int32_t npairs,
+__global__ void gather_dist_kernel(const PairMeta* pairs, const td_t* td, int32_t npairs,
                                    int32_t* dist) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < npairs) dist[i] = td[pairs[i].td_off];
 }
 
-void launch_gather_dist(const PairMeta* pairs, const int32_t* td, int32_t npairs, int32_t* dist,
+void launch_gather_dist(const PairMeta* pairs, const td_t* td, int32_t npairs, int32_t* dist,
                         hipStream_t s) {
   if (npairs <= 0) return;
   hipLaunchKernelGGL(gather_dist_kernel, dim3((npairs + 255) / 256), dim3(256), 0, s, pairs, td,
                      npairs, dist);
+}
+
+__global__ __launch_bounds__(256) void widen_td_kernel(const td_t* __restrict__ src,
+                                                       int32_t* __restrict__ dst, int64_t n) {
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step)
+    dst[i] = src[i];
+}
+
+void launch_widen_td(const td_t* src, int32_t* dst, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(widen_td_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
 }
 
 }  // namespace xted
