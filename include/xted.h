@@ -208,12 +208,16 @@ int xted_set_batch_groups(xted_context* ctx, int32_t groups, int32_t split_min);
  * offsets move the tables of shallow pairs into the levels whose time is set
  * by the few large tables of the deepest pairs. */
 int xted_set_level_align(xted_context* ctx, int32_t mode);
-/* on = 1 (the default): the tables of each launch start in four size bands,
- * longest first (by rows), so a launch ends on short tables; 0: in plan
- * order. The reference sorts each level's queue by table size, ascending
- * (thrust::sort_by_key, X-TED_GPU/parallel_X_TED.cu:374). Results do not
- * depend on it. */
-int xted_set_size_bands(xted_context* ctx, int32_t on);
+/* mode 1 (the default): the tables of each launch start in four size
+ * bands, longest first (by rows), so a launch ends on short tables; 2: in
+ * addition, the row-scan tables of a level and class that need more slot
+ * storage than a workgroup's share of LDS at full occupancy take a launch of
+ * their own, so they do not lower the occupancy of the others; 0: one
+ * launch per level and class, tables in plan order. The reference sorts
+ * each level's queue by table size, ascending (thrust::sort_by_key,
+ * X-TED_GPU/parallel_X_TED.cu:374). Results do not depend on it. Other
+ * modes: XTED_ERR_INVALID_ARG. */
+int xted_set_size_bands(xted_context* ctx, int32_t mode);
 /* on = 1 (the default): the level launches of a batch without
  * column-blocked tables are captured once as a HIP graph and re-launched
  * while a later batch plans to the same launches (same counts per level and

@@ -10,9 +10,10 @@ pair's levels sit in the batch and the order of a launch's tables. This
 module times a workload under a staged grid of rules on the device it runs
 on:
 
-  1. the schedule: level alignment x size bands;
+  1. the schedule: level alignment x size bands (none, by rows, by rows and
+     light / heavy slot storage);
   2. the tail rule;
-  3. a coarse grid of slot bytes per wave x columns per lane;
+  3. a coarse grid of size bands x slot bytes per wave x columns per lane;
   4. a fine grid of slot bytes per wave (1 KiB steps) around the best.
 
 The workload is the 4096-pair uneven batch's generator on a seed stream of
@@ -41,11 +42,13 @@ DEFAULT = {"lds_per_wave": 10 * 1024, "max_cpl": 8, "min_waves": 1, "wide_cols":
            "tail_pairs": 32, "level_align": 0, "size_bands": 1}
 # Seed stream of the tuning workload (bench.py times stream 3).
 SEED_TUNE = 1003
-COARSE_LDS = [6 * 1024, 10 * 1024, 14 * 1024, 20 * 1024]
+COARSE_LDS = [6 * 1024, 10 * 1024, 14 * 1024, 20 * 1024, 28 * 1024]
 STAGES = [
-    {"level_align": [0, 1, 2], "size_bands": [0, 1]},
+    {"level_align": [0, 1, 2], "size_bands": [0, 1, 2]},
     {"tail_pairs": [0, 8, 32]},
-    {"lds_per_wave": COARSE_LDS, "max_cpl": [4, 8, 16]},
+    # Slot bytes per wave set which tables share a launch, and with it how
+    # far splitting light from heavy tables raises occupancy: searched together.
+    {"size_bands": [1, 2], "lds_per_wave": COARSE_LDS, "max_cpl": [8, 16]},
 ]
 FINE_STEP = 1024
 FINE_SPAN = 3  # steps either side of the coarse best

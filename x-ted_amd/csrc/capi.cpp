@@ -508,9 +508,13 @@ int xted_set_batch_groups(xted_context* ctx, int32_t groups, int32_t split_min) 
   return OK;
 }
 
-int xted_set_size_bands(xted_context* ctx, int32_t on) {
+int xted_set_size_bands(xted_context* ctx, int32_t mode) {
   if (!ctx) return ERR_INVALID_ARG;
-  ctx->ctx.size_bands = on ? 1 : 0;
+  if (mode < 0 || mode > 2) {
+    set_error("set_size_bands: mode must be 0, 1 or 2");
+    return ERR_INVALID_ARG;
+  }
+  ctx->ctx.size_bands = mode;
   return OK;
 }
 

@@ -136,16 +136,33 @@ constexpr int kFirstSeqXLForm = 2;
 constexpr int kMaxImports = 256;
 XTED_HD inline int row_class(int wi, int ci) { return 1 + wi * 5 + ci; }
 
-// Each (level, class) bucket is cut into kBands size bands, largest tables
-// first, so that a launch starts its longest tables first and its last
-// workgroups are short ones (column-blocked classes use band 0 only). The
-// plan's buckets are (level, class, band), and one launch covers the bands
-// of a (level, class).
-constexpr int kBands = 4;
-XTED_HD inline int size_band(int cls, int rows) {
+// Each (level, class) bucket is cut into size bands, largest tables first,
+// so that a launch starts its longest tables first and its last workgroups
+// are short ones (column-blocked classes use band 0 only). The plan's
+// buckets are (level, class, band), and one launch covers kRowBands
+// consecutive bands.
+//
+// A launch gives every workgroup the largest slot storage of its tables,
+// and LDS is what bounds how many workgroups a CU holds. In the 4096-pair
+// batch a W = 2 launch's tables need 2 KiB of slot rows on average and its
+// largest 28 KiB, which held a CU to 5 workgroups where registers allow 10.
+// So with mode 2 the row-scan tables that need more than light_lds(W) (the
+// LDS share of one workgroup at the register-bound occupancy of 20 waves
+// per CU) go to bands of their own (kRowBands .. 2*kRowBands-1), and a
+// (level, class) takes two launches: light tables and heavy ones.
+constexpr int kRowBands = 4;
+constexpr int kBands = 2 * kRowBands;
+XTED_HD inline int64_t light_lds(int cls) {
+  const int w = 1 << ((cls - 1) / 5);
+  return (int64_t)w * 8192 - 512;
+}
+// mode 0: band 0 (plan order); 1: row bands; 2: row bands, light / heavy.
+XTED_HD inline int size_band(int cls, int rows, int64_t lds, int mode) {
+  if (mode == 0) return 0;
   if (cls == kClassTiny) return rows >= 13 ? 0 : rows >= 9 ? 1 : rows >= 5 ? 2 : 3;
   if (cls < 1 || cls > kNumRowClasses) return 0;
-  return rows >= 256 ? 0 : rows >= 64 ? 1 : rows >= 16 ? 2 : 3;
+  const int rb = rows >= 256 ? 0 : rows >= 64 ? 1 : rows >= 16 ? 2 : 3;
+  return mode == 2 && lds > light_lds(cls) ? kRowBands + rb : rb;
 }
 XTED_HD inline int bucket_of(int level, int cls, int band) {
   return (level * kNumClasses + cls) * kBands + band;

@@ -676,6 +676,7 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
     unsigned long long off, cnt, cells;
     int32_t lds;      // largest slot storage over the bands
     int level;
+    bool heavy;       // the tables that need more slot storage than light_lds
   };
   std::vector<Launch> lv, leaf;
   for (int gi = 0; gi < ng; ++gi) {
@@ -692,26 +693,32 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
     // tables with placeholders, so it runs before the group's first level.
     if (L == 0 && g.lines[g.npairs] > 0 && !g.leaf_early)
       leaf.push_back({&g, kClassLane, -1, 0, (unsigned long long)g.lines[g.npairs],
-                      (unsigned long long)g.leaf_cells, 0, -1});
+                      (unsigned long long)g.leaf_cells, 0, -1, false});
     unsigned long long off = g.level_off;
     for (int c = 0; c < kNumClasses; ++c) {
-      // The bands of (L, c) are contiguous in the task array, largest first.
-      unsigned long long cnt = 0, cells = 0;
-      int32_t lds = 0;
-      for (int bd = 0; bd < kBands; ++bd) {
-        const size_t b = bucket_of(L, c, bd);
-        cnt += g.h_counts[b];
-        cells += g.h_cells[b];
-        lds = std::max(lds, g.h_maxlds[b]);
+      // The bands of (L, c) are contiguous in the task array, largest first:
+      // light tables, then heavy ones (size_band), one launch each.
+      for (int half = 0; half < kBands / kRowBands; ++half) {
+        unsigned long long cnt = 0, cells = 0;
+        int32_t lds = 0;
+        for (int bd = half * kRowBands; bd < (half + 1) * kRowBands; ++bd) {
+          const size_t b = bucket_of(L, c, bd);
+          cnt += g.h_counts[b];
+          cells += g.h_cells[b];
+          lds = std::max(lds, g.h_maxlds[b]);
+        }
+        if (cnt) lv.push_back({&g, c, bucket_of(L, c, 0), off, cnt, cells, lds, L, half > 0});
+        off += cnt;
       }
-      if (cnt) lv.push_back({&g, c, bucket_of(L, c, 0), off, cnt, cells, lds, L});
-      off += cnt;
     }
     g.level_off = off;
     ++g.next_level;
   }
-  std::stable_sort(lv.begin(), lv.end(),
-                   [](const Launch& x, const Launch& y) { return x.cells > y.cells; });
+  // Heavy launches first (few long tables at low occupancy: each takes a
+  // stream of its own where there are enough), then by cells.
+  std::stable_sort(lv.begin(), lv.end(), [](const Launch& x, const Launch& y) {
+    return x.heavy != y.heavy ? x.heavy : x.cells > y.cells;
+  });
   if (prof) {
     while (prof_ev_.size() < 2 * (prof->size() + lv.size() + leaf.size())) {
       hipEvent_t e;

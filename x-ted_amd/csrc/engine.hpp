@@ -259,8 +259,9 @@ class Context {
   // choice moves shallow pairs' tables into levels whose time is set by the
   // few large tables of the deepest pairs.
   int32_t level_align = 0;
-  // 1: the tables of a launch start largest first (size bands, kBands);
-  // 0: in plan order.
+  // 1: the tables of a launch start largest first (size bands); 2: and the
+  // tables that need more slot storage than light_lds take launches of
+  // their own; 0: plan order.
   int32_t size_bands = 1;
   ClassProfile class_profile[kNumClasses];
   std::vector<LaunchRecord> launch_log;  // every timed launch of the last profiled call

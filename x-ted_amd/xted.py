@@ -371,9 +371,11 @@ class Context:
     def set_graphs(self, on=True):
         _check(lib().xted_set_graphs(self._h, 1 if on else 0))
 
-    def set_size_bands(self, on=True):
-        """Tables of a launch start longest first (four size bands)."""
-        _check(lib().xted_set_size_bands(self._h, 1 if on else 0))
+    def set_size_bands(self, mode=1):
+        """1: tables of a launch start longest first (four size bands); 2: and
+        tables that need more slot storage than a workgroup's share of LDS at
+        full occupancy take launches of their own; 0 (or False): plan order."""
+        _check(lib().xted_set_size_bands(self._h, int(mode)))
 
     def set_level_align(self, mode):
         """0: every pair starts at level 0; 1: every pair ends at the last
