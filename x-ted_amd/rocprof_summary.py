@@ -136,7 +136,7 @@ def resources(src, hipcc="/opt/rocm/bin/hipcc", arch="gfx950"):
             base = f"xl_bw{64 * w * c}" if xl else f"rowscan_w{w}_c{c}"
             return base + ("_cost" if cost else "")
         m = re.search(r"N4xted\d+(\w+?)E", mangled) or re.search(r"4xted(\d+)(\w+)", mangled)
-        for name in ("leaf_kernel", "tiny_kernel", "plan_kernel", "tree_prep_kernel",
+        for name in ("leaf_cost_kernel", "leaf_kernel", "tiny_kernel", "plan_kernel", "tree_prep_kernel",
                      "cost_prefix_kernel", "xl_layout_kernel", "xl_zero_kernel",
                      "plan_scan_kernel", "plan_clear_kernel", "gather_dist_kernel"):
             if name in mangled:
