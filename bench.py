@@ -194,6 +194,11 @@ def xl_pair_bench(reps=3):
         _, _, pst = ctx.batch(xs, ys)
         prof = ctx.class_profile()
         launches = ctx.launch_profile()
+        # And one with a single level stream (each launch alone): the
+        # critical path of the pair's 38 levels.
+        ctx.set_level_streams(1)
+        ctx.batch(xs, ys)
+        launches1 = ctx.launch_profile()
     wall, st, dist = sorted(runs, key=lambda r: r[0])[len(runs) // 2]
     xl = {xted.class_name(c): round(v[3], 2) for c, v in prof.items()
           if 27 <= c < 27 + len(xted.XL_BW)}
@@ -219,6 +224,7 @@ def xl_pair_bench(reps=3):
             "column_blocked_ms": xl, "classes": classes,
             "root_table": {"class": xted.class_name(root["cls"]), "ms": round(root["ms"], 3),
                            "rows": 20000, "us_per_row": round(root["ms"] * 1e3 / 20000, 3)},
+            "level_profile": level_profile(launches1, launches),
             "checked": "distance against the oracle (tests/golden/xl_expected.json)"}
 
 

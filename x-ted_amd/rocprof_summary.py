@@ -237,6 +237,26 @@ def traffic(kernel, fetch, write, cfetch, cwrite, copy_bytes):
     ])
 
 
+def levels(bench, key):
+    """Per-level table of a bench.py JSON line's level_profile (the batch's
+    at the top level, or under key, e.g. xl_pair)."""
+    with open(bench) as f:
+        line = [x for x in f.read().splitlines() if x.startswith("{")][-1]
+    d = json.loads(line)
+    lp = (d[key] if key else d)["level_profile"]
+    out = [f"levels: {lp['levels']}",
+           f"critical_path_ms (sum over levels of the longest launch, one stream): {lp['critical_path_ms']}",
+           f"level_spans_ms_4_streams: {lp['level_spans_ms_4_streams']}",
+           f"dp_span_ms_4_streams: {lp['dp_span_ms_4_streams']}",
+           f"thin_levels (fewer workgroups than half the CUs): {lp['thin_levels']}, "
+           f"{lp['thin_level_ms_4_streams']} ms, fraction {lp['thin_fraction']}",
+           f"{'level':>6}{'tables':>12}{'longest_1s_ms':>15}{'span_4s_ms':>12}"]
+    for L, (t, lo, sp) in enumerate(zip(lp["per_level_tables"], lp["per_level_longest_ms_1_stream"],
+                                        lp["per_level_ms_4_streams"])):
+        out.append(f"{L:>6}{t:>12}{lo:>15.3f}{sp:>12.3f}")
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -258,8 +278,13 @@ def main():
     t.add_argument("copy_fetch")
     t.add_argument("copy_write")
     t.add_argument("--copy-bytes", type=int, required=True)
+    lv = sub.add_parser("levels", help="per-level table from a bench.py JSON line")
+    lv.add_argument("bench")
+    lv.add_argument("--key", default=None, help="e.g. xl_pair for the 20k pair's profile")
     a = ap.parse_args()
-    if a.cmd == "resources":
+    if a.cmd == "levels":
+        print(levels(a.bench, a.key))
+    elif a.cmd == "resources":
         print(resources(a.source))
     elif a.cmd == "kernels":
         print(kernels(a.trace, a.bench))
