@@ -218,6 +218,11 @@ int xted_set_level_align(xted_context* ctx, int32_t mode);
  * X-TED_GPU/parallel_X_TED.cu:374). Results do not depend on it. Other
  * modes: XTED_ERR_INVALID_ARG. */
 int xted_set_size_bands(xted_context* ctx, int32_t mode);
+/* Size-band mode 2: a row-scan table of a W-wave form is heavy when its slot
+ * rows need more than W * bytes_per_wave bytes of LDS (default 10240: a
+ * workgroup's share at 16 waves per CU). Bytes outside [1024, 153600]:
+ * XTED_ERR_INVALID_ARG. Results do not depend on it. */
+int xted_set_light_lds(xted_context* ctx, int32_t bytes_per_wave);
 /* on = 1 (the default): the level launches of a batch without
  * column-blocked tables are captured once as a HIP graph and re-launched
  * while a later batch plans to the same launches (same counts per level and

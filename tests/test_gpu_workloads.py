@@ -96,6 +96,25 @@ def test_batch4096_plan_order(batch4096):
     assert len(bad) == 0, f"{len(bad)} distances differ, first pairs {bad[:8].tolist()}"
 
 
+@pytest.mark.parametrize("light", [2048, 65536])
+def test_batch4096_light_heavy_split(batch4096, light):
+    """Size-band mode 2 with most row-scan tables heavy (2 KiB per wave) or
+    none (64 KiB): every distance against the oracle's."""
+    ctx, trees, xs, ys = batch4096
+    expect = np.load(os.path.join(GOLDEN, "batch4096_dist.npy"))
+    rule = autotune.load()
+    ctx.set_size_bands(2)
+    ctx.set_light_lds(light)
+    try:
+        print(f"batch4096: light/heavy split at {light} bytes per wave", flush=True)
+        d, _, _ = ctx.batch(xs, ys)
+    finally:
+        ctx.set_size_bands(rule["size_bands"])
+        ctx.set_light_lds(rule["light_lds"])
+    bad = np.flatnonzero(d != expect)
+    assert len(bad) == 0, f"{len(bad)} distances differ, first pairs {bad[:8].tolist()}"
+
+
 def test_batch4096_sample_against_live_oracle(batch4096_sample):
     ctx, trees, xs, ys, sample, d = batch4096_sample
     for k in sample:

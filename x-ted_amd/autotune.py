@@ -4,7 +4,7 @@ The reference fixes its granularity thresholds at 15 / 18,750 / 25,250,000
 cells (X-TED_GPU/parallel_X_TED.cu:382) from a cost model fitted to one GPU
 (supplement, section 2). Here the class rule has knobs (include/xted.h,
 xted_set_tuning / xted_set_tail_rule / xted_set_level_align /
-xted_set_size_bands): slot bytes per wave before a table gets more waves,
+xted_set_size_bands / xted_set_light_lds): slot bytes per wave before a table gets more waves,
 the most columns per lane, a latency rule for the last levels, where each
 pair's levels sit in the batch and the order of a launch's tables. This
 module times a workload under a staged grid of rules on the device it runs
@@ -14,7 +14,8 @@ on:
      light / heavy slot storage);
   2. the tail rule;
   3. a coarse grid of size bands x slot bytes per wave x columns per lane;
-  4. a fine grid of slot bytes per wave (1 KiB steps) around the best.
+  4. the slot bytes per wave above which a table is heavy (size bands 2);
+  5. a fine grid of slot bytes per wave (1 KiB steps) around the best.
 
 The workload is the 4096-pair uneven batch's generator on a seed stream of
 its own (not the one bench.py times). The rule starts as the default; a
@@ -39,7 +40,7 @@ sys.path.insert(0, HERE)
 import xted  # noqa: E402
 
 DEFAULT = {"lds_per_wave": 10 * 1024, "max_cpl": 8, "min_waves": 1, "wide_cols": 1 << 30,
-           "tail_pairs": 32, "level_align": 0, "size_bands": 1}
+           "tail_pairs": 32, "level_align": 0, "size_bands": 1, "light_lds": 10 * 1024}
 # Seed stream of the tuning workload (bench.py times stream 3).
 SEED_TUNE = 1003
 COARSE_LDS = [6 * 1024, 10 * 1024, 14 * 1024, 20 * 1024, 28 * 1024]
@@ -49,6 +50,8 @@ STAGES = [
     # Slot bytes per wave set which tables share a launch, and with it how
     # far splitting light from heavy tables raises occupancy: searched together.
     {"size_bands": [1, 2], "lds_per_wave": COARSE_LDS, "max_cpl": [8, 16]},
+    # Where heavy tables start (size-band mode 2 only).
+    {"light_lds": [6 * 1024, 8 * 1024, 10 * 1024, 12 * 1024, 14 * 1024]},
 ]
 FINE_STEP = 1024
 FINE_SPAN = 3  # steps either side of the coarse best
@@ -60,6 +63,7 @@ def apply(ctx, rule):
     ctx.set_tail_rule(rule["tail_pairs"])
     ctx.set_level_align(rule["level_align"])
     ctx.set_size_bands(rule["size_bands"])
+    ctx.set_light_lds(rule["light_lds"])
 
 
 def load(path=TUNING_FILE):

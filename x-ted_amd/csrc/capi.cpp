@@ -518,6 +518,17 @@ int xted_set_size_bands(xted_context* ctx, int32_t mode) {
   return OK;
 }
 
+int xted_set_light_lds(xted_context* ctx, int32_t bytes_per_wave) {
+  if (!ctx) return ERR_INVALID_ARG;
+  if (bytes_per_wave < 1024 || bytes_per_wave > kLdsMaxTable) {
+    set_error("set_light_lds: bytes per wave must be in [1024, " + std::to_string(kLdsMaxTable) +
+              "]");
+    return ERR_INVALID_ARG;
+  }
+  ctx->ctx.light_per_wave = bytes_per_wave;
+  return OK;
+}
+
 int xted_set_level_align(xted_context* ctx, int32_t mode) {
   if (!ctx || mode < 0 || mode > 2) {
     set_error("xted_set_level_align: mode must be 0, 1 or 2");

@@ -146,23 +146,21 @@ XTED_HD inline int row_class(int wi, int ci) { return 1 + wi * 5 + ci; }
 // and LDS is what bounds how many workgroups a CU holds. In the 4096-pair
 // batch a W = 2 launch's tables need 2 KiB of slot rows on average and its
 // largest 28 KiB, which held a CU to 5 workgroups where registers allow 10.
-// So with mode 2 the row-scan tables that need more than light_lds(W) (the
-// LDS share of one workgroup at the register-bound occupancy of 20 waves
-// per CU) go to bands of their own (kRowBands .. 2*kRowBands-1), and a
-// (level, class) takes two launches: light tables and heavy ones.
+// So with mode 2 the row-scan tables that need more than light_per_wave
+// bytes per wave (8 KiB is one workgroup's share of LDS at 20 waves per CU)
+// go to bands of their own (kRowBands .. 2*kRowBands-1), and a (level,
+// class) takes two launches: light tables and heavy ones.
 constexpr int kRowBands = 4;
 constexpr int kBands = 2 * kRowBands;
-XTED_HD inline int64_t light_lds(int cls) {
-  const int w = 1 << ((cls - 1) / 5);
-  return (int64_t)w * 8192 - 512;
-}
+constexpr int kLightPerWave = 10 * 1024;  // default of Context::light_per_wave
 // mode 0: band 0 (plan order); 1: row bands; 2: row bands, light / heavy.
-XTED_HD inline int size_band(int cls, int rows, int64_t lds, int mode) {
+XTED_HD inline int size_band(int cls, int rows, int64_t lds, int mode, int light_per_wave) {
   if (mode == 0) return 0;
   if (cls == kClassTiny) return rows >= 13 ? 0 : rows >= 9 ? 1 : rows >= 5 ? 2 : 3;
   if (cls < 1 || cls > kNumRowClasses) return 0;
   const int rb = rows >= 256 ? 0 : rows >= 64 ? 1 : rows >= 16 ? 2 : 3;
-  return mode == 2 && lds > light_lds(cls) ? kRowBands + rb : rb;
+  const int64_t light = (int64_t)(1 << ((cls - 1) / 5)) * light_per_wave;
+  return mode == 2 && lds > light ? kRowBands + rb : rb;
 }
 XTED_HD inline int bucket_of(int level, int cls, int band) {
   return (level * kNumClasses + cls) * kBands + band;

@@ -587,6 +587,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   pa.tail_level = g.tail_lv.as<uint8_t>();
   pa.force_class = force_class;
   pa.bands = size_bands;
+  pa.light_per_wave = light_per_wave;
   pa.lane_enable = lane_enable;
   pa.pair_leaf = g.leaf_d.as<uint8_t>();
   pa.counts = g.counts.as<unsigned long long>();

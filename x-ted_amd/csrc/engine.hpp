@@ -263,6 +263,8 @@ class Context {
   // tables that need more slot storage than light_lds take launches of
   // their own; 0: plan order.
   int32_t size_bands = 1;
+  // Mode 2: slot bytes per wave above which a row-scan table is heavy.
+  int32_t light_per_wave = kLightPerWave;
   ClassProfile class_profile[kNumClasses];
   std::vector<LaunchRecord> launch_log;  // every timed launch of the last profiled call
 

@@ -66,6 +66,7 @@ struct PlanArgs {
   const uint8_t* tail_level;   // nlevels
   int32_t force_class;         // -1 = automatic; else row-scan tables go to this class if they fit
   int32_t bands;               // 0: every table in band 0 (plan order within a launch)
+  int32_t light_per_wave;      // bands 2: heavier tables take launches of their own
   int32_t lane_enable;         // 0 = single-row tables use the row-scan kernels
   const uint8_t* pair_leaf;    // per pair: its lines are in closed form (null: lane_enable)
   unsigned long long* counts;  // per bucket (bucket_of: level, class, band)

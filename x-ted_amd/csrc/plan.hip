@@ -82,7 +82,7 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
       need = row_need;
     }
   }
-  *bucket = bucket_of(level, cls, size_band(cls, a <= b ? a : b, need, A.bands));
+  *bucket = bucket_of(level, cls, size_band(cls, a <= b ? a : b, need, A.bands, A.light_per_wave));
   *lds = (int)need;
   *i0 = (uint32_t)A.pool.skr[kx];
   *j0 = (uint32_t)A.pool.skr[ly];

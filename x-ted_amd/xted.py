@@ -81,6 +81,7 @@ _SIGNATURES = {
     "xted_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_level_align": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_size_bands": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "xted_set_light_lds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_bracket_to_lines": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64,
                                              ctypes.c_char_p, ctypes.c_int64,
                                              ctypes.POINTER(ctypes.c_int64),
@@ -376,6 +377,10 @@ class Context:
         tables that need more slot storage than a workgroup's share of LDS at
         full occupancy take launches of their own; 0 (or False): plan order."""
         _check(lib().xted_set_size_bands(self._h, int(mode)))
+
+    def set_light_lds(self, bytes_per_wave):
+        """Size-band mode 2: slot bytes per wave above which a table is heavy."""
+        _check(lib().xted_set_light_lds(self._h, int(bytes_per_wave)))
 
     def set_level_align(self, mode):
         """0: every pair starts at level 0; 1: every pair ends at the last
