@@ -67,6 +67,14 @@ struct alignas(16) TaskDesc {
 };
 static_assert(sizeof(TaskDesc) == 64, "TaskDesc is one 64-byte read");
 
+// What a forest kernel reads of one node, in one 16-byte load: as a column
+// of a table (crec: x = leaf rank at orl(i), y = leaf rank of i-1) or as a
+// row (rrec: x = kdep(i), y = kdep(i-1)); bit 31 of y is set when i-1 is a
+// leaf. Entry 0's y is 0.
+struct alignas(16) NodeRec {
+  int32_t orl, label, x, y;
+};
+
 // Pointers to the pool arrays (device addresses).
 struct PoolView {
   const int32_t* parent;
@@ -88,6 +96,8 @@ struct PoolView {
   const int32_t* rtop;   // keyroot at the top of node's right path
   const int32_t* pdel;   // cost mode: deletion costs of nodes [0, i) (n+1 entries per tree)
   const int32_t* pins;   // cost mode: insertion costs of nodes [0, i)
+  const NodeRec* crec;   // per node, as a table column
+  const NodeRec* rrec;   // per node, as a table row
   const TreeMeta* trees;
 };
 

@@ -104,7 +104,8 @@ Context::~Context() {
   na_.release();
   for (DevBuf* b : {&parent_, &label_, &orl_, &kdep_, &kdn_, &lrank_, &kr_, &krkd_, &krsize_,
                     &krleaf_, &trees_, &prep_scratch_, &prep_off_, &skr_, &skd_, &ssize_, &sleaf_,
-                    &spre_, &kdoff_, &rtop_, &cdel_, &cins_, &cren_, &pdel_, &pins_, &ctot_})
+                    &spre_, &kdoff_, &rtop_, &cdel_, &cins_, &cren_, &pdel_, &pins_, &ctot_, &crec_,
+                    &rrec_})
     b->release();
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
@@ -177,6 +178,8 @@ PoolView Context::view() const {
   v.rtop = rtop_.as<int32_t>();
   v.pdel = pdel_.as<int32_t>();
   v.pins = pins_.as<int32_t>();
+  v.crec = crec_.as<NodeRec>();
+  v.rrec = rrec_.as<NodeRec>();
   v.trees = trees_.as<TreeMeta>();
   return v;
 }
@@ -187,6 +190,8 @@ int Context::grow_pool(int64_t nodes) {
                     &krleaf_, &skr_, &skd_, &ssize_, &sleaf_, &spre_, &kdoff_, &rtop_, &pdel_,
                     &pins_})
     ST_TRY(b->grow(bytes, true, stream_));
+  for (DevBuf* b : {&crec_, &rrec_})
+    ST_TRY(b->grow((size_t)nodes * sizeof(NodeRec), true, stream_));
   return OK;
 }
 
@@ -287,6 +292,8 @@ int Context::pool_add(int32_t ntrees, const int32_t* sizes, const int32_t* paren
   pa.spre = spre_.as<int32_t>();
   pa.kdoff = kdoff_.as<int32_t>();
   pa.rtop = rtop_.as<int32_t>();
+  pa.crec = crec_.as<NodeRec>();
+  pa.rrec = rrec_.as<NodeRec>();
   pa.trees = trees_.as<TreeMeta>();
   pa.scratch = prep_scratch_.as<int32_t>();
   pa.scratch_off = prep_off_.as<int64_t>();

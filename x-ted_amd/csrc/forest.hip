@@ -529,12 +529,9 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   const bool rows_x = dsc.base_rx < 0;
   const int64_t offR = dsc.offR;
   const int64_t offC = dsc.offC;
-  const int32_t* __restrict__ orlR = A.pool.orl + offR;
-  const int32_t* __restrict__ labR = A.pool.label + offR;
-  const int32_t* __restrict__ kdepR = A.pool.kdep + offR;
+  const NodeRec* __restrict__ rrecR = A.pool.rrec + offR;
   const int32_t* __restrict__ lrankR = A.pool.lrank + offR;
-  const int32_t* __restrict__ orlC = A.pool.orl + offC;
-  const int32_t* __restrict__ labC = A.pool.label + offC;
+  const NodeRec* __restrict__ crecC = A.pool.crec + offC;
   const int32_t* __restrict__ lrankC = A.pool.lrank + offC;
   const int32_t* __restrict__ kdepC = A.pool.kdep + offC;
   const int32_t* __restrict__ rtopC = A.pool.rtop + offC;
@@ -656,16 +653,19 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       const int j = g * E + e;
       const int32_t t = TCOL(g, e);
       const int32_t c = max(cmax - 1 - t, c0);
-      const int32_t oc = orlC[c];
-      const int32_t lab = labC[c];
-      const int32_t lro = lrankC[oc];
-      const int32_t ocm = orlC[max(c - 1, c0)];
-      const int32_t lrm = lrankC[max(c - 1, c0)];
+      // One record: orl(c), label, leaf rank at orl(c), leaf rank of c-1
+      // (bit 31: c-1 is a leaf; only read when c > c0).
+      const NodeRec cr = crecC[c];
+      const int32_t oc = cr.orl;
+      const int32_t lab = cr.label;
+      const int32_t lro = cr.x;
+      const int32_t lrm = cr.y & 0x7FFFFFFF;
+      const bool prev_leaf = cr.y < 0;
       const bool tree = (oc == orl_c0);
       tcol[j] = tree;
       lb[j] = tree ? lab : kCost ? hc - Hpre[oc + 1] : cmax - oc - 1;
       foff[j] = tree ? wb : lro - lbase;
-      const bool writes = t < ncols && c > c0 && ocm == c - 1;
+      const bool writes = t < ncols && c > c0 && prev_leaf;
       woff[j] = writes ? lrm - lbase : sinkix;
       // F[rmax][c] + 1 - t = cmax - c + 1 - t with unit costs, shifted
       Fpm[j] = fpm0 + gsh(g);
@@ -760,10 +760,11 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     const int32_t rr = rmax - 1 - (64 * q + lane);
     const int32_t r = max(rr, r0);  // rows past the table read row r0
     const int32_t rp = max(r - 1, r0);
-    const int32_t orr = orlR[r], lab = labR[r], kd = kdepR[r];
-    const int32_t orp = orlR[rp], kdp = kdepR[rp];
+    const NodeRec rc = rrecR[r];  // orl(r), label, kdep(r), kdep(r-1) | r-1 a leaf
+    const int32_t orr = rc.orl, lab = rc.label, kd = rc.x;
+    const int32_t kdp = rc.y & 0x7FFFFFFF;
     const int32_t fs = (orr + 1 < rmax) ? kd - base - 1 : 0;
-    const bool saved = r > r0 && orp == r - 1;
+    const bool saved = r > r0 && rc.y < 0;
     const int32_t ws = saved ? (((kdp - base - 1) << 16) | (int32_t)0x80000000) : 0;
     mlab = lab;
     mrf = orr + 1;
@@ -1163,11 +1164,8 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
   const TaskDesc dsc = A.descs[live ? tix : 0];  // idle segments redo task 0, storing nothing
   const int64_t offR = dsc.offR;
   const int64_t offC = dsc.offC;
-  const int32_t* __restrict__ orlR = A.pool.orl + offR;
-  const int32_t* __restrict__ labR = A.pool.label + offR;
-  const int32_t* __restrict__ kdepR = A.pool.kdep + offR;
-  const int32_t* __restrict__ orlC = A.pool.orl + offC;
-  const int32_t* __restrict__ labC = A.pool.label + offC;
+  const NodeRec* __restrict__ rrecR = A.pool.rrec + offR;
+  const NodeRec* __restrict__ crecC = A.pool.crec + offC;
   const int32_t* __restrict__ lrankC = A.pool.lrank + offC;
   const int32_t r0 = dsc.r0, c0 = dsc.c0;
   td_t* TDo = A.td + dsc.tdo;  // [r][c]
@@ -1197,15 +1195,15 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
 
   // This lane's column (as in the row scan, one element).
   const int32_t c = max(cmax - 1 - t, c0);
-  const int32_t oc = orlC[c];
-  const int32_t labc = labC[c];
-  const int32_t ocm = orlC[max(c - 1, c0)];
+  const NodeRec cr = crecC[c];  // orl(c), label, leaf rank at orl(c), leaf rank of c-1 | c-1 a leaf
+  const int32_t oc = cr.orl;
+  const int32_t labc = cr.label;
   const bool tree_col = oc == cmax - 1;
   const int32_t bfar = tree_col ? -1 : kCost ? hc - Hpre[oc + 1] : cmax - oc - 1;
-  const int32_t foff = tree_col ? wb : lrankC[oc] - lbase;
+  const int32_t foff = tree_col ? wb : cr.x - lbase;
   const bool col_ok = live && t < ncols;
-  const bool writes = col_ok && c > c0 && ocm == c - 1;
-  uint16_t* const wdst_rel = writes ? slots + (lrankC[max(c - 1, c0)] - lbase) : sink + threadIdx.x;
+  const bool writes = col_ok && c > c0 && cr.y < 0;
+  uint16_t* const wdst_rel = writes ? slots + ((cr.y & 0x7FFFFFFF) - lbase) : sink + threadIdx.x;
   int32_t negt = kCost ? Hpre[c] - hq : -t;
   asm("" : "+v"(negt));
 
@@ -1213,12 +1211,12 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
   int32_t mlab, mrf, mslot, mvp = 0;
   {
     const int32_t r = max(rmax - 1 - t, r0);
-    const int32_t rp = max(r - 1, r0);
-    const int32_t orr = orlR[r], kd = kdepR[r];
-    const int32_t orp = orlR[rp], kdp = kdepR[rp];
+    const NodeRec rc = rrecR[r];  // orl(r), label, kdep(r), kdep(r-1) | r-1 a leaf
+    const int32_t orr = rc.orl, kd = rc.x;
+    const int32_t kdp = rc.y & 0x7FFFFFFF;
     const int32_t fs = (orr + 1 < rmax) ? kd - base - 1 : 0;
-    const bool saved = r > r0 && orp == r - 1;
-    mlab = labR[r];
+    const bool saved = r > r0 && rc.y < 0;
+    mlab = rc.label;
     mrf = orr + 1;
     mslot = fs | (saved ? (((kdp - base - 1) << 16) | (int32_t)0x80000000) : 0);
     if constexpr (kCost) mvp = Vpre[r];

@@ -25,6 +25,8 @@ struct PrepArgs {
   int32_t* spre;
   int32_t* kdoff;
   int32_t* rtop;
+  NodeRec* crec;
+  NodeRec* rrec;
   TreeMeta* trees;
   int32_t* scratch;            // 6*(n+1) ints per tree at scratch_off[b]
   const int64_t* scratch_off;  // per block of this launch

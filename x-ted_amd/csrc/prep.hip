@@ -204,6 +204,14 @@ __global__ __launch_bounds__(kPrepThreads) void tree_prep_kernel(PrepArgs A) {
   }
   __syncthreads();
 
+  // ---- per-node records of the forest kernels ------------------------------
+  for (int i = tid; i < n; i += kPrepThreads) {
+    const int32_t prev_leaf = i > 0 && orl[i - 1] == i - 1 ? (int32_t)0x80000000 : 0;
+    A.crec[off + i] = NodeRec{orl[i], label[i], lrank[orl[i]], (i > 0 ? lrank[i - 1] : 0) | prev_leaf};
+    A.rrec[off + i] = NodeRec{orl[i], label[i], kdep[i], (i > 0 ? kdep[i - 1] : 0) | prev_leaf};
+  }
+  __syncthreads();
+
   // ---- keyroots sorted for the scheduler -----------------------------------
   // Leaf keyroots first, then the other keyroots by ascending nesting height
   // and, within a height, by subtree size band (<= 16, 64, 256, 1024, more
