@@ -75,6 +75,12 @@ struct alignas(16) NodeRec {
   int32_t orl, label, x, y;
 };
 
+// Under a cost model, a node's deletion / insertion cost prefix sums at
+// itself and just past its subtree (at orl(i) + 1), in one 16-byte load.
+struct alignas(16) CostRec {
+  int32_t del, ins, del_end, ins_end;
+};
+
 // Pointers to the pool arrays (device addresses).
 struct PoolView {
   const int32_t* parent;
@@ -98,6 +104,7 @@ struct PoolView {
   const int32_t* pins;   // cost mode: insertion costs of nodes [0, i)
   const NodeRec* crec;   // per node, as a table column
   const NodeRec* rrec;   // per node, as a table row
+  const CostRec* ccrec;  // cost mode: per node, pdel / pins at it and past its subtree
   const TreeMeta* trees;
 };
 

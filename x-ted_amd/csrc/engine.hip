@@ -105,7 +105,7 @@ Context::~Context() {
   for (DevBuf* b : {&parent_, &label_, &orl_, &kdep_, &kdn_, &lrank_, &kr_, &krkd_, &krsize_,
                     &krleaf_, &trees_, &prep_scratch_, &prep_off_, &skr_, &skd_, &ssize_, &sleaf_,
                     &spre_, &kdoff_, &rtop_, &cdel_, &cins_, &cren_, &pdel_, &pins_, &ctot_, &crec_,
-                    &rrec_})
+                    &rrec_, &ccrec_})
     b->release();
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
@@ -180,6 +180,7 @@ PoolView Context::view() const {
   v.pins = pins_.as<int32_t>();
   v.crec = crec_.as<NodeRec>();
   v.rrec = rrec_.as<NodeRec>();
+  v.ccrec = ccrec_.as<CostRec>();
   v.trees = trees_.as<TreeMeta>();
   return v;
 }
@@ -192,6 +193,7 @@ int Context::grow_pool(int64_t nodes) {
     ST_TRY(b->grow(bytes, true, stream_));
   for (DevBuf* b : {&crec_, &rrec_})
     ST_TRY(b->grow((size_t)nodes * sizeof(NodeRec), true, stream_));
+  ST_TRY(ccrec_.grow((size_t)nodes * sizeof(CostRec), true, stream_));
   return OK;
 }
 
@@ -324,6 +326,8 @@ int Context::cost_prefix(int32_t first, int32_t ntrees) {
   ca.pdel = pdel_.as<int32_t>();
   ca.pins = pins_.as<int32_t>();
   ca.totals = ctot_.as<int32_t>();
+  ca.orl = orl_.as<int32_t>();
+  ca.rec = ccrec_.as<CostRec>();
   ca.first_tree = first;
   ca.alphabet = cost_alphabet_;
   launch_cost_prefix(ca, ntrees, stream_);

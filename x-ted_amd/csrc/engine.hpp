@@ -285,7 +285,7 @@ class Context {
   std::vector<HostTreeInfo> info_;
   int64_t pool_nodes_ = 0;  // used entries (sum of n+1)
   DevBuf parent_, label_, orl_, kdep_, kdn_, lrank_, kr_, krkd_, krsize_, krleaf_, trees_;
-  DevBuf skr_, skd_, ssize_, sleaf_, spre_, kdoff_, rtop_, crec_, rrec_;
+  DevBuf skr_, skd_, ssize_, sleaf_, spre_, kdoff_, rtop_, crec_, rrec_, ccrec_;
   DevBuf prep_scratch_, prep_off_;
   Group batch_[kMaxLevelStreams];  // the groups run_batch reuses
   Group na_;                        // run_batch_noalloc's group, bound to na_arena_

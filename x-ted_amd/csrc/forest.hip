@@ -663,14 +663,21 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       const bool prev_leaf = cr.y < 0;
       const bool tree = (oc == orl_c0);
       tcol[j] = tree;
-      lb[j] = tree ? lab : kCost ? hc - Hpre[oc + 1] : cmax - oc - 1;
+      // (kCost) the column tree's horizontal sums at c and at orl(c) + 1
+      int32_t hcol = 0, hfar = 0;
+      if constexpr (kCost) {
+        const CostRec cc = A.pool.ccrec[offC + c];
+        hcol = rows_x ? cc.ins : cc.del;
+        hfar = rows_x ? cc.ins_end : cc.del_end;
+      }
+      lb[j] = tree ? lab : kCost ? hc - hfar : cmax - oc - 1;
       foff[j] = tree ? wb : lro - lbase;
       const bool writes = t < ncols && c > c0 && prev_leaf;
       woff[j] = writes ? lrm - lbase : sinkix;
       // F[rmax][c] + 1 - t = cmax - c + 1 - t with unit costs, shifted
       Fpm[j] = fpm0 + gsh(g);
       if constexpr (kCost) {
-        negt[j] = Hpre[c] - hq;
+        negt[j] = hcol - hq;
         // Opaque to the optimiser, so that sums with it stay three-operand
         // adds (v_add3_u32) instead of becoming an add and a subtract.
         asm("" : "+v"(negt[j]));
@@ -770,7 +777,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
     mrf = orr + 1;
     mslot = fs | ws;
     if constexpr (kXL) mrid = (saved && rr >= r0) ? lrankR[rp] - lrankR[r0] : -1;
-    if constexpr (kCost) mvp = Vpre[r];
+    if constexpr (kCost) {
+      const CostRec cr = A.pool.ccrec[offR + r];
+      mvp = rows_x ? cr.del : cr.ins;  // Vpre[r]
+    }
   };
   const int32_t nchunks = (nrows + 63) >> 6;
   int32_t mlab, mrf, mslot, mrid = -1, nlab = 0, nrf = 0, nslot = 0, nrid = -1;
@@ -1199,12 +1209,18 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
   const int32_t oc = cr.orl;
   const int32_t labc = cr.label;
   const bool tree_col = oc == cmax - 1;
-  const int32_t bfar = tree_col ? -1 : kCost ? hc - Hpre[oc + 1] : cmax - oc - 1;
+  int32_t hcol = 0, hfar = 0;  // (kCost) Hpre[c], Hpre[oc + 1]
+  if constexpr (kCost) {
+    const CostRec cc = A.pool.ccrec[offC + c];
+    hcol = rows_x ? cc.ins : cc.del;
+    hfar = rows_x ? cc.ins_end : cc.del_end;
+  }
+  const int32_t bfar = tree_col ? -1 : kCost ? hc - hfar : cmax - oc - 1;
   const int32_t foff = tree_col ? wb : cr.x - lbase;
   const bool col_ok = live && t < ncols;
   const bool writes = col_ok && c > c0 && cr.y < 0;
   uint16_t* const wdst_rel = writes ? slots + ((cr.y & 0x7FFFFFFF) - lbase) : sink + threadIdx.x;
-  int32_t negt = kCost ? Hpre[c] - hq : -t;
+  int32_t negt = kCost ? hcol - hq : -t;
   asm("" : "+v"(negt));
 
   // Lane t's metadata: row rmax-1-t (rows past the table read row r0).
@@ -1219,7 +1235,10 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
     mlab = rc.label;
     mrf = orr + 1;
     mslot = fs | (saved ? (((kdp - base - 1) << 16) | (int32_t)0x80000000) : 0);
-    if constexpr (kCost) mvp = Vpre[r];
+    if constexpr (kCost) {
+      const CostRec cr = A.pool.ccrec[offR + r];
+      mvp = rows_x ? cr.del : cr.ins;  // Vpre[r]
+    }
   }
   // TD of this lane's column for every row (row k at r = rmax-1-k).
   int32_t tdv[kTinySeg];

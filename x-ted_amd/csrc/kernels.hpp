@@ -45,6 +45,8 @@ struct CostPrefixArgs {
   int32_t* pdel;
   int32_t* pins;
   int32_t* totals;     // 2 per tree of this launch
+  const int32_t* orl;  // the pool's
+  CostRec* rec;        // per node: the sums at i and at orl(i) + 1
   int32_t first_tree;
   int32_t alphabet;    // labels outside [0, alphabet) read the last entry
 };

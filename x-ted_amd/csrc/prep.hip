@@ -300,6 +300,13 @@ __global__ __launch_bounds__(kPrepThreads) void cost_prefix_kernel(CostPrefixArg
     }
     __syncthreads();
   }
+  const int32_t* orl = A.orl + meta.node_off;
+  const int32_t* pd = A.pdel + meta.node_off;
+  const int32_t* pi = A.pins + meta.node_off;
+  for (int i = threadIdx.x; i < n; i += kPrepThreads) {
+    const int32_t e = orl[i] + 1;
+    A.rec[meta.node_off + i] = CostRec{pd[i], pi[i], pd[e], pi[e]};
+  }
 }
 
 void launch_cost_prefix(const CostPrefixArgs& a, int ntrees, hipStream_t s) {
