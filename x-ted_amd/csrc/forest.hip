@@ -502,6 +502,10 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   constexpr int BW = 64 * W * C;                    // columns of one block
   constexpr int KQ = kXL ? kMaxImports / 64 : 1;    // import entries per lane
   constexpr bool kSeqForm = kXL && W * C <= 16;      // may sweep all blocks itself
+  // Unit costs keep row k's values less k (F - t + T_g - k): the vertical
+  // step F[r+1][c] + 1 of row k+1 is then row k's value itself, with no add
+  // per column and row; the terms read from the far rows and TD take the -k
+  // through their (scalar) per-column constant, and stores add k back.
   extern __shared__ uint16_t slots[];
   // Per-wave scan totals, double-buffered by row parity: a wave may write
   // row r-1's total while a slower wave still reads row r's.
@@ -920,6 +924,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       if (k >= nrows) break;
       const int32_t i = k & 63;  // index in the metadata chunk
       const int32_t r = rmax - 1 - k;
+      const int32_t ku = kCost ? 0 : k;  // the unit form's row shift
       load_row(max(r - PD, r0), tdq[(u + PD) % S]);
       if constexpr (kXL) {
         constexpr int32_t kM = kXLEvery - 1, kHalf = kXLEvery / 2;
@@ -1003,11 +1008,11 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
                 rename = (int32_t)(labr != lb[j]);
               }
               const int32_t diag = fprm + rename;
-              const int32_t far = lb[j] + tdv(u, j) + term(j);
+              const int32_t far = lb[j] + tdv(u, j) + (term(j) - ku);
               // Blended rather than branched per element.
               dm = tcol[j] ? diag : far;
             } else {
-              dm = sv[j] + tdv(u, j) + term(j);
+              dm = sv[j] + tdv(u, j) + (term(j) - ku);
             }
             h[j] = min(dm, Fpm[j]);
             if (e) h[j] = min(h[j], h[j - 1]);
@@ -1025,7 +1030,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       // Carry into this wave: the boundary F[r][cmax] - (-1), or for a
       // later column block the carry of the blocks before it, then the
       // totals of this workgroup's earlier waves.
-      int32_t carry = kCost ? frc + (hc - hq) : (rmax - r) + 1;
+      int32_t carry = kCost ? frc + (hc - hq) : 2;  // unit: (rmax - r) + 1 - k
       if constexpr (kXL) {
         if (blk > 0) {
           const uint32_t clo = __builtin_amdgcn_readlane(cwl, i);
@@ -1054,8 +1059,9 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
       // wave's last element ends with h = min(its carry-in, its own total) =
       // this wave's carry-in, and F = h + t; for wave 0 of block 0 this gives
       // F[r][cmax] = rmax - r.
-      left_prev = carry + qleft;
-      // h becomes G - t; Fpm = G + 1 - t = h + 1.
+      left_prev = carry + qleft - (kCost ? 0 : 1);  // unit: in row k+1's shift
+      // h becomes G - t; Fpm = G + 1 - t = h + 1 (unit costs: row k+1 is shifted
+      // by one more, so Fpm = h).
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const int32_t cg = min(carry, pre[g]) + gsh(g);
@@ -1063,7 +1069,7 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
         for (int e = 0; e < E; ++e) {
           const int j = g * E + e;
           h[j] = min(cg, h[j]);
-          Fpm[j] = h[j] + vnext;
+          Fpm[j] = kCost ? h[j] + vnext : h[j];
         }
       }
 
@@ -1082,15 +1088,15 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           const int32_t c = cb - ((j / E) * 64 * E + j % E);
           if (c >= c0 && tcol[j]) {
             // c * ldt < 2^30: trees have at most 32767 nodes
-            rowp[c] = (td_t)(h[j] - term(j));
-            colp[(uint32_t)c * ldt32] = (td_t)(h[j] - term(j));
+            rowp[c] = (td_t)(h[j] - term(j) + ku);
+            colp[(uint32_t)c * ldt32] = (td_t)(h[j] - term(j) + ku);
           }
         }
       }
       if (wrow) {
         // Columns this lane does not save write the row's sink entry.
 #pragma unroll
-        for (int j = 0; j < C; ++j) wrow_ptr[woff[j]] = (uint16_t)(h[j] - term(j));
+        for (int j = 0; j < C; ++j) wrow_ptr[woff[j]] = (uint16_t)(h[j] - term(j) + ku);
         if (threadIdx.x == 0) wrow_ptr[wb] = (uint16_t)frc;
         if constexpr (kXL) {
           // Imported far values of this saved row; far values later
@@ -1114,7 +1120,8 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
           }
 #pragma unroll
           for (int j = 0; j < C; ++j) {
-            if (xinf[j] >= 0) xl_store(xword(rid, blk, xinf[j]), A.xl_epoch, h[j] - term(j), seq);
+            if (xinf[j] >= 0)
+              xl_store(xword(rid, blk, xinf[j]), A.xl_epoch, h[j] - term(j) + ku, seq);
           }
         }
         // The next row may read this slot row at columns of other waves.
