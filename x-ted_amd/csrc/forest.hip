@@ -607,7 +607,8 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   const int32_t sinkix = wb + 1 + kds;
   const int32_t rowlen = sinkix + 1;
 
-  const int wave = threadIdx.x >> 6;
+  // Uniform over the wave, so it lives in a scalar register.
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int32_t tlane = tb0 + wave * 64 * C + lane * E;  // t of this lane's element (0, 0)
 #define TCOL(g, e) (tlane + (g) * 64 * E + (e))
