@@ -146,6 +146,13 @@ def test_invalid_inputs(gpu_ctx):
     with pytest.raises(xted.XtedError) as e:
         gpu_ctx.ted(big, ok)
     assert e.value.status == xted.ERR_TOO_LARGE
+    # schedule knobs out of range are refused and leave the context as it was
+    for call, arg in ((gpu_ctx.set_size_bands, 3), (gpu_ctx.set_size_bands, -1),
+                      (gpu_ctx.set_light_lds, 512), (gpu_ctx.set_light_lds, 1 << 20),
+                      (gpu_ctx.set_level_align, 3), (gpu_ctx.set_level_streams, 0)):
+        with pytest.raises(xted.XtedError) as e:
+            call(arg)
+        assert e.value.status == xted.ERR_INVALID_ARG
     # the context still works after errors
     assert gpu_ctx.ted(ok, ok)[0] == 0
 
