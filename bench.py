@@ -384,7 +384,9 @@ def level_profile(log1, log4):
             d["end"] = max(d["end"], r["start_ms"] + r["ms"])
             d["longest"] = max(d["longest"], r["ms"])
             d["tables"] += r["tables"]
-            d["wgs"] += -(-r["tables"] // 16) if r["cls"] == xted.CLASS_TINY else r["tables"]
+            tiny = r["cls"] - xted.CLASS_TINY
+            per_wg = 256 // xted.TINY_SEG[tiny] if 0 <= tiny < len(xted.TINY_SEG) else 1
+            d["wgs"] += -(-r["tables"] // per_wg)
         return lv
     l1, l4 = per_level(log1), per_level(log4)
     span4 = {L: d["end"] - d["start"] for L, d in l4.items()}

@@ -42,9 +42,9 @@ extern "C" {
  * of 4096, 2048, 1024, 512, 256 or 128 columns, one workgroup per block, for
  * tables too wide or too LDS-hungry for one workgroup (the role of the
  * reference's multi-block and last-table kernels,
- * X-TED_GPU/parallel_X_TED.cu:883-1131); 33 = tables of at most 16 columns,
- * four to a wave (the role of the reference's single-thread tasks,
- * X-TED_GPU/parallel_X_TED.cu:382). */
+ * X-TED_GPU/parallel_X_TED.cu:883-1131); 33, 34, 35 = tables of at most 16,
+ * 32 or 64 columns, four, two or one to a wave (the role of the reference's
+ * single-thread tasks, X-TED_GPU/parallel_X_TED.cu:382). */
 #define XTED_CLASS_AUTO (-1)
 #define XTED_CLASS_LANE 0
 
@@ -171,7 +171,7 @@ int xted_all_pairs(xted_context* ctx, int32_t ntrees, const int32_t* sizes,
 int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t max_cpl,
                     int32_t min_waves, int32_t wide_cols);
 /* Forces every table that fits into class cls, a row-scan class (1..25), a
- * column-blocked one (27..32) or the tiny form (33); XTED_CLASS_AUTO
+ * column-blocked one (27..32) or a tiny form (33..35); XTED_CLASS_AUTO
  * restores the rule.
  * lane_enable = 0 sends single-row tables to the row-scan kernels. */
 int xted_force_class(xted_context* ctx, int32_t cls, int32_t lane_enable);
@@ -223,6 +223,14 @@ int xted_set_size_bands(xted_context* ctx, int32_t mode);
  * workgroup's share at 16 waves per CU). Bytes outside [1024, 153600]:
  * XTED_ERR_INVALID_ARG. Results do not depend on it. */
 int xted_set_light_lds(xted_context* ctx, int32_t bytes_per_wave);
+/* Tiny forms in use: bit k (k = 0, 1, 2) lets tables of at most 16 << k
+ * columns whose slot rows fit in 2048 << k bytes take class 33 + k, one table
+ * per (16 << k)-lane segment of a wave; a table takes the narrowest form in
+ * use that holds it, and other tables take the row-scan forms.
+ * Default 1 (16 columns). The reference runs every table with one
+ * thread per cell anti-diagonal (X-TED_GPU/parallel_X_TED.cu:883-1131).
+ * Results do not depend on it. A mask outside [0, 7]: XTED_ERR_INVALID_ARG. */
+int xted_set_tiny_forms(xted_context* ctx, int32_t mask);
 /* on = 1 (the default): the level launches of a batch without
  * column-blocked tables are captured once as a HIP graph and re-launched
  * while a later batch plans to the same launches (same counts per level and

@@ -119,17 +119,20 @@ def test_every_column_blocked_form_forced(cost_ctx, cls):
         assert_same(cost_ctx, x, y, costs, threads=16)
 
 
+@pytest.mark.parametrize("form", range(3))
 @pytest.mark.parametrize("seed", range(4))
-def test_tiny_class_forced(cost_ctx, seed):
-    """Every table of at most 16 columns in the tiny form's cost variant (four
-    tables per wave, one per 16-lane row), including the one-row tables of
-    leaf keyroots, which the cost model plans like any other."""
+def test_tiny_class_forced(cost_ctx, seed, form):
+    """Every table of at most 16, 32 or 64 columns in the cost variant of
+    tiny form 0, 1 or 2 (four, two or one table per wave, one per 16-, 32- or
+    64-lane segment), including the one-row tables of leaf keyroots, which
+    the cost model plans like any other."""
     rng = np.random.default_rng(500 + seed)
     costs = rand_costs(rng, 5, zeros=seed % 2 == 1)
     cost_ctx.set_costs(costs)
-    cost_ctx.force_class(xted.CLASS_TINY, lane_enable=False)
-    for i, (kx, ky, n1, n2) in enumerate([(0, 2, 60, 50), (1, 1, 90, 120), (2, 0, 16, 16),
-                                          (0, 0, 1, 30)]):
+    cost_ctx.force_class(xted.CLASS_TINY + form, lane_enable=False)
+    seg = xted.TINY_SEG[form]
+    for i, (kx, ky, n1, n2) in enumerate([(0, 2, 60, 50), (1, 1, 90, 120), (2, 0, seg, seg),
+                                          (0, 0, 1, 30), (1, 2, seg + 1, seg - 1)]):
         x = gen_any(kx, n1, 5, 500 + seed, 2 * i)
         y = gen_any(ky, n2, 5, 500 + seed, 2 * i + 1)
         assert_same(cost_ctx, x, y, costs)

@@ -59,7 +59,7 @@ def test_1000_node_mixed_shapes(gpu_ctx):
     assert_same(gpu_ctx, y, x, threads=16)
 
 
-@pytest.mark.parametrize("cls", list(range(1, 26)) + [xted.CLASS_TINY])
+@pytest.mark.parametrize("cls", list(range(1, 26)) + [xted.CLASS_TINY + k for k in range(3)])
 def test_every_row_class_forced(gpu_ctx, cls):
     """Every table that fits goes through one (waves, columns-per-lane) class;
     single-row tables too (lane kernel off). Results must not change a bit."""
@@ -149,7 +149,8 @@ def test_invalid_inputs(gpu_ctx):
     # schedule knobs out of range are refused and leave the context as it was
     for call, arg in ((gpu_ctx.set_size_bands, 3), (gpu_ctx.set_size_bands, -1),
                       (gpu_ctx.set_light_lds, 512), (gpu_ctx.set_light_lds, 1 << 20),
-                      (gpu_ctx.set_level_align, 3), (gpu_ctx.set_level_streams, 0)):
+                      (gpu_ctx.set_level_align, 3), (gpu_ctx.set_level_streams, 0),
+                      (gpu_ctx.set_tiny_forms, -1), (gpu_ctx.set_tiny_forms, 8)):
         with pytest.raises(xted.XtedError) as e:
             call(arg)
         assert e.value.status == xted.ERR_INVALID_ARG

@@ -115,6 +115,23 @@ def test_batch4096_light_heavy_split(batch4096, light):
     assert len(bad) == 0, f"{len(bad)} distances differ, first pairs {bad[:8].tolist()}"
 
 
+@pytest.mark.parametrize("mask", [0, 1, 7])
+def test_batch4096_tiny_forms(batch4096, mask):
+    """The auto rule with no tiny form, the 16-lane one only, or all three
+    (tables of up to 64 columns one per wave): every distance against the
+    oracle's."""
+    ctx, trees, xs, ys = batch4096
+    expect = np.load(os.path.join(GOLDEN, "batch4096_dist.npy"))
+    ctx.set_tiny_forms(mask)
+    try:
+        print(f"batch4096: tiny forms {mask}", flush=True)
+        d, _, _ = ctx.batch(xs, ys)
+    finally:
+        ctx.set_tiny_forms(autotune.load()["tiny_forms"])
+    bad = np.flatnonzero(d != expect)
+    assert len(bad) == 0, f"{len(bad)} distances differ, first pairs {bad[:8].tolist()}"
+
+
 def test_batch4096_sample_against_live_oracle(batch4096_sample):
     ctx, trees, xs, ys, sample, d = batch4096_sample
     for k in sample:

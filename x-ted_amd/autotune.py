@@ -4,9 +4,10 @@ The reference fixes its granularity thresholds at 15 / 18,750 / 25,250,000
 cells (X-TED_GPU/parallel_X_TED.cu:382) from a cost model fitted to one GPU
 (supplement, section 2). Here the class rule has knobs (include/xted.h,
 xted_set_tuning / xted_set_tail_rule / xted_set_level_align /
-xted_set_size_bands / xted_set_light_lds): slot bytes per wave before a table gets more waves,
-the most columns per lane, a latency rule for the last levels, where each
-pair's levels sit in the batch and the order of a launch's tables. This
+xted_set_size_bands / xted_set_light_lds / xted_set_tiny_forms): slot bytes per wave before a
+table gets more waves, the most columns per lane, a latency rule for the last levels, where each
+pair's levels sit in the batch, the order of a launch's tables and the widest tables the tiny
+forms take. This
 module times a workload under a staged grid of rules on the device it runs
 on:
 
@@ -15,7 +16,8 @@ on:
   2. the tail rule;
   3. a coarse grid of size bands x slot bytes per wave x columns per lane;
   4. the slot bytes per wave above which a table is heavy (size bands 2);
-  5. a fine grid of slot bytes per wave (1 KiB steps) around the best.
+  5. the tiny forms (tables of up to 16, 32 or 64 columns);
+  6. a fine grid of slot bytes per wave (1 KiB steps) around the best.
 
 The workload is the 4096-pair uneven batch's generator on a seed stream of
 its own (not the one bench.py times). The rule starts as the default; a
@@ -40,7 +42,8 @@ sys.path.insert(0, HERE)
 import xted  # noqa: E402
 
 DEFAULT = {"lds_per_wave": 10 * 1024, "max_cpl": 8, "min_waves": 1, "wide_cols": 1 << 30,
-           "tail_pairs": 32, "level_align": 0, "size_bands": 1, "light_lds": 10 * 1024}
+           "tail_pairs": 32, "level_align": 0, "size_bands": 1, "light_lds": 10 * 1024,
+           "tiny_forms": 1}
 # Seed stream of the tuning workload (bench.py times stream 3).
 SEED_TUNE = 1003
 COARSE_LDS = [6 * 1024, 10 * 1024, 14 * 1024, 20 * 1024, 28 * 1024]
@@ -52,6 +55,8 @@ STAGES = [
     {"size_bands": [1, 2], "lds_per_wave": COARSE_LDS, "max_cpl": [8, 16]},
     # Where heavy tables start (size-band mode 2 only).
     {"light_lds": [6 * 1024, 8 * 1024, 10 * 1024, 12 * 1024, 14 * 1024]},
+    # Which tables the tiny forms take (16-lane only, up to 32 or 64 columns).
+    {"tiny_forms": [1, 3, 7]},
 ]
 FINE_STEP = 1024
 FINE_SPAN = 3  # steps either side of the coarse best
@@ -64,6 +69,7 @@ def apply(ctx, rule):
     ctx.set_level_align(rule["level_align"])
     ctx.set_size_bands(rule["size_bands"])
     ctx.set_light_lds(rule["light_lds"])
+    ctx.set_tiny_forms(rule["tiny_forms"])
 
 
 def load(path=TUNING_FILE):

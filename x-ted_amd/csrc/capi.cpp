@@ -529,6 +529,17 @@ int xted_set_light_lds(xted_context* ctx, int32_t bytes_per_wave) {
   return OK;
 }
 
+int xted_set_tiny_forms(xted_context* ctx, int32_t mask) {
+  if (!ctx) return ERR_INVALID_ARG;
+  if (mask < 0 || mask >= (1 << kNumTinyForms)) {
+    set_error("set_tiny_forms: mask must be in [0, " + std::to_string((1 << kNumTinyForms) - 1) +
+              "]");
+    return ERR_INVALID_ARG;
+  }
+  ctx->ctx.rule.tiny = mask;
+  return OK;
+}
+
 int xted_set_level_align(xted_context* ctx, int32_t mode) {
   if (!ctx || mode < 0 || mode > 2) {
     set_error("xted_set_level_align: mode must be 0, 1 or 2");

@@ -128,17 +128,21 @@ XTED_HD inline uint32_t task_j0(uint64_t t) { return (uint32_t)t & 0xFFFFFu; }
 // Classes 27..32: column-blocked form for tables too wide or too LDS-hungry
 //   for one workgroup: blocks of xl_bw(k) columns, one workgroup each,
 //   chained per row through device memory, class = 27 + k.
-// Class 33: tables of at most kTinyCols columns, four to a wave (one per
-//   16-lane row of it).
+// Classes 33..35: tiny tables of at most 16, 32 or 64 columns, one per 16-,
+//   32- or 64-lane segment of a wave (four, two or one to a wave).
 constexpr int kClassLane = 0;  // the leaf class
 constexpr int kNumRowClasses = 25;
 constexpr int kClassXL = 26;
 constexpr int kFirstXLClass = 27;
 constexpr int kNumXLForms = 6;
-constexpr int kClassTiny = kFirstXLClass + kNumXLForms;  // 33
-constexpr int kNumClasses = kClassTiny + 1;              // 34
-constexpr int kTinyCols = 16;
-constexpr int kTinyMaxLds = 2048;  // slot bytes of one tiny table (16 of them per workgroup)
+constexpr int kClassTiny = kFirstXLClass + kNumXLForms;  // 33: the first tiny form
+constexpr int kNumTinyForms = 3;
+constexpr int kNumClasses = kClassTiny + kNumTinyForms;  // 36
+XTED_HD inline bool is_tiny(int cls) { return cls >= kClassTiny && cls < kNumClasses; }
+// Lanes per table (and most columns) of tiny form k.
+XTED_HD inline int tiny_seg(int k) { return 16 << k; }
+// Slot bytes of one table of tiny form k: 32 KiB per workgroup of 256 lanes.
+XTED_HD inline int64_t tiny_max_lds(int k) { return (int64_t)2048 << k; }
 // Column-blocked forms: waves and columns per lane of one block.
 XTED_HD inline int xl_waves(int k) { return k <= 1 ? 8 : k == 2 ? 4 : 2; }
 XTED_HD inline int xl_cpl(int k) { return k == 0 ? 8 : k <= 3 ? 4 : k == 4 ? 2 : 1; }
@@ -173,7 +177,10 @@ constexpr int kLightPerWave = 10 * 1024;  // default of Context::light_per_wave
 // mode 0: band 0 (plan order); 1: row bands; 2: row bands, light / heavy.
 XTED_HD inline int size_band(int cls, int rows, int64_t lds, int mode, int light_per_wave) {
   if (mode == 0) return 0;
-  if (cls == kClassTiny) return rows >= 13 ? 0 : rows >= 9 ? 1 : rows >= 5 ? 2 : 3;
+  if (is_tiny(cls)) {
+    const int q = tiny_seg(cls - kClassTiny) / 4;  // row bands of a quarter segment
+    return rows > 3 * q ? 0 : rows > 2 * q ? 1 : rows > q ? 2 : 3;
+  }
   if (cls < 1 || cls > kNumRowClasses) return 0;
   const int rb = rows >= 256 ? 0 : rows >= 64 ? 1 : rows >= 16 ? 2 : 3;
   const int64_t light = (int64_t)(1 << ((cls - 1) / 5)) * light_per_wave;
@@ -193,6 +200,12 @@ constexpr int kMaxCols = 64 * 16 * kMaxRowWaves;  // 16384
 // 16 columns per lane at 16 waves would spill); wider tables are column-blocked.
 constexpr int kMaxOnChipCols = 64 * 8 * kMaxRowWaves;  // 8192
 
+// 16-lane form only: the 32-lane form ran the same 4096-pair batch in the
+// same time (100.6 against 100.9 ms unit, 125.0 against 124.4 ms under a
+// cost model) and the 64-lane one 4% slower (4 waves per SIMD for its 64
+// TD registers per lane).
+constexpr int32_t kTinyDefault = 1;
+
 // Tuning knobs carried by the plan builder (autotuner writes these).
 struct ClassRule {
   int32_t lds_per_wave;  // bytes of slot storage one wave may own
@@ -201,7 +214,9 @@ struct ClassRule {
   int32_t lds_max;       // slot bytes one workgroup may hold (device limit)
   int32_t max_cpl;       // most columns per lane before a table gets more waves
   int32_t xl_seq_rows = kXLSeqRows;  // column-blocked tables this short: one workgroup
-  int32_t tiny = 1;                  // tables of <= kTinyCols columns take the tiny form
+  // Tiny forms in use (bit k: tables of at most tiny_seg(k) columns take
+  // form k, the narrowest that holds them).
+  int32_t tiny = kTinyDefault;
   // Widest form a pipelined column-blocked table may use (0: 4096-column
   // blocks, 1: 2048, 2: 1024, ...): narrower blocks put more workgroups on
   // one table.
@@ -247,7 +262,9 @@ XTED_HD inline int table_class(int a, int b, int64_t lds_need, int kd_r, int kd_
     }
     return kClassXL;
   }
-  if (rule.tiny && cols <= kTinyCols && lds_need <= kTinyMaxLds) return kClassTiny;
+  for (int k = 0; k < kNumTinyForms; ++k)
+    if ((rule.tiny >> k & 1) && cols <= tiny_seg(k) && lds_need <= tiny_max_lds(k))
+      return kClassTiny + k;
   int wi = 0;
   while (wi < 4 && lds_need > (int64_t)rule.lds_per_wave * (1 << wi)) ++wi;
   while (wi < 4 && cols > 64 * rule.max_cpl * (1 << wi)) ++wi;

@@ -595,6 +595,7 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   pa.rule = rule;
   pa.rule_tail = ClassRule{lds_limit, 1, 1 << 30, lds_limit, 16, rule.xl_seq_rows};
   pa.rule_tail.xl_first = rule.xl_first;
+  pa.rule_tail.tiny = rule.tiny;
   pa.tail_level = g.tail_lv.as<uint8_t>();
   pa.force_class = force_class;
   pa.bands = size_bands;
@@ -780,7 +781,7 @@ int Context::launch_levels(Group* const* gs, int ng, std::vector<ProfLaunch>* pr
       da.ren = cren_.as<int32_t>();
       da.alphabet = cost_alphabet_;
       da.costs = cost_alphabet_ > 0 ? 1 : 0;
-      if (w.cls >= kFirstXLClass) {
+      if (w.cls >= kFirstXLClass && w.cls < kFirstXLClass + kNumXLForms) {
         const int bw = xl_bw(w.cls - kFirstXLClass);
         da.xl_off = g.xl_off.as<int64_t>() + g.xl_first[w.bucket];
         da.xl_ticket = g.tickets.as<unsigned int>() + w.bucket;

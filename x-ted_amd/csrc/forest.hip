@@ -1143,41 +1143,69 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// Tiny tables (at most 16 columns, so at most 16 rows): four per wave, one
-// per 16-lane DPP row, one column per lane (t = lane & 15). A 16-lane row is
-// what row_shr scans and row_newbcast broadcasts over, so a table needs no
-// barrier and no LDS exchange; lane k of a table holds the metadata of its
-// row k and row k takes it with row_newbcast:k; the TD values of all rows
-// are read up front. Tables of one wave run to the longest one's row
-// count, each lane storing only while its table has rows left.
+// Tiny tables (at most kSeg = 16, 32 or 64 columns, so at most kSeg rows):
+// 64 / kSeg per wave, one per kSeg-lane segment, one column per lane
+// (t = lane % kSeg). A segment is one or more whole 16-lane DPP rows, so the
+// row's scan and shift stay in registers (row_shr, row_bcast, wave_shr) and a
+// table needs no barrier and no LDS exchange. Row metadata sits in lanes:
+// lane t of a table holds that of rows t % 16 + 16 s in set s (each 16-lane
+// row of a segment holds every set), and row k takes set k / 16 with
+// row_newbcast:(k % 16). The TD values of all rows are read up front. Tables
+// of one wave run to the longest one's row count, each lane storing only
+// while its table has rows left.
 // ---------------------------------------------------------------------------
 template <class F, int... K>
 __device__ __forceinline__ void static_for(F& f, std::integer_sequence<int, K...>) {
   (f(std::integral_constant<int, K>{}), ...);
 }
 
-constexpr int kTinySeg = 16;
-constexpr int kTinyPerGroup = 256 / kTinySeg;  // tables per workgroup of 4 waves
 #define DPP_ROW_NEWBCAST(n) (0x150 + (n))
 
-__device__ __forceinline__ int seg16_incl_min(int v) {
-  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(1), 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(2), 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(4), 0xf, 0xf, false));
-  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(8), 0xf, 0xf, false));
-  return v;
+// Inclusive prefix minimum over each kSeg-lane segment (lane order).
+template <int kSeg>
+__device__ __forceinline__ int seg_incl_min(int v) {
+  if constexpr (kSeg == 64) {
+    return wave_incl_min(v);
+  } else {
+    v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(1), 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(2), 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(4), 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_SHR(8), 0xf, 0xf, false));
+    if constexpr (kSeg == 32)
+      v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, DPP_ROW_BCAST15, 0xa, 0xf, false));
+    return v;
+  }
+}
+
+// Value of the previous lane of the segment (its lane t = 0 receives fill).
+template <int kSeg>
+__device__ __forceinline__ int seg_shr1(int v, int fill, int t) {
+  if constexpr (kSeg == 16) {
+    return __builtin_amdgcn_update_dpp(fill, v, DPP_ROW_SHR(1), 0xf, 0xf, false);
+  } else {
+    const int u = wave_shr1(v, fill);
+    return kSeg == 64 || t != 0 ? u : fill;
+  }
+}
+
+// Value of set k / 16 in lane k % 16 of this lane's 16-lane row.
+template <int k, int kSets>
+__device__ __forceinline__ int32_t row_meta(const int32_t (&m)[kSets]) {
+  return __builtin_amdgcn_update_dpp(0, m[k / 16], DPP_ROW_NEWBCAST(k % 16), 0xf, 0xf, false);
 }
 
 // kCost: the cost form, with the substitutions of the row scan (Q(c) from
 // the column tree's prefix sums in place of t, the row node's vertical cost
 // in place of 1, a rename gather on tree cells).
-template <bool kCost>
+template <int kSeg, bool kCost>
 __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) {
+  constexpr int kPerGroup = 256 / kSeg;  // tables per workgroup of 4 waves
+  constexpr int kSets = kSeg / 16;       // metadata registers per lane
   extern __shared__ uint16_t tslots[];
   __shared__ uint16_t sink[256];
-  const int t = threadIdx.x & (kTinySeg - 1);  // this lane's column, t = cmax-1-c
-  const int seg = threadIdx.x / kTinySeg;      // table of this lane in the workgroup
-  const int64_t tix = (int64_t)blockIdx.x * kTinyPerGroup + seg;
+  const int t = threadIdx.x & (kSeg - 1);  // this lane's column, t = cmax-1-c
+  const int seg = threadIdx.x / kSeg;      // table of this lane in the workgroup
+  const int64_t tix = (int64_t)blockIdx.x * kPerGroup + seg;
   const bool live = tix < A.ntasks;
   const TaskDesc dsc = A.descs[live ? tix : 0];  // idle segments redo task 0, storing nothing
   const int64_t offR = dsc.offR;
@@ -1231,33 +1259,35 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
   int32_t negt = kCost ? hcol - hq : -t;
   asm("" : "+v"(negt));
 
-  // Lane t's metadata: row rmax-1-t (rows past the table read row r0).
-  int32_t mlab, mrf, mslot, mvp = 0;
-  {
-    const int32_t r = max(rmax - 1 - t, r0);
+  // Metadata of rows t % 16 + 16 s (rows past the table read row r0).
+  int32_t mlab[kSets], mrf[kSets], mslot[kSets], mvp[kSets];
+#pragma unroll
+  for (int s = 0; s < kSets; ++s) {
+    const int32_t r = max(rmax - 1 - ((t & 15) + 16 * s), r0);
     const NodeRec rc = rrecR[r];  // orl(r), label, kdep(r), kdep(r-1) | r-1 a leaf
     const int32_t orr = rc.orl, kd = rc.x;
     const int32_t kdp = rc.y & 0x7FFFFFFF;
     const int32_t fs = (orr + 1 < rmax) ? kd - base - 1 : 0;
     const bool saved = r > r0 && rc.y < 0;
-    mlab = rc.label;
-    mrf = orr + 1;
-    mslot = fs | (saved ? (((kdp - base - 1) << 16) | (int32_t)0x80000000) : 0);
+    mlab[s] = rc.label;
+    mrf[s] = orr + 1;
+    mslot[s] = fs | (saved ? (((kdp - base - 1) << 16) | (int32_t)0x80000000) : 0);
+    mvp[s] = 0;
     if constexpr (kCost) {
-      const CostRec cr = A.pool.ccrec[offR + r];
-      mvp = rows_x ? cr.del : cr.ins;  // Vpre[r]
+      const CostRec vc = A.pool.ccrec[offR + r];
+      mvp[s] = rows_x ? vc.del : vc.ins;  // Vpre[r]
     }
   }
   // TD of this lane's column for every row (row k at r = rmax-1-k).
-  int32_t tdv[kTinySeg];
+  int32_t tdv[kSeg];
 #pragma unroll
-  for (int k = 0; k < kTinySeg; ++k)
+  for (int k = 0; k < kSeg; ++k)
     tdv[k] = k < nrows ? TDo[(int64_t)(rmax - 1 - k) * ldo + c] : 0;
 
   // Rows of the longest table of this wave.
-  const int32_t nmax =
-      max(max(__builtin_amdgcn_readlane(nrows, 0), __builtin_amdgcn_readlane(nrows, 16)),
-          max(__builtin_amdgcn_readlane(nrows, 32), __builtin_amdgcn_readlane(nrows, 48)));
+  int32_t nmax = __builtin_amdgcn_readlane(nrows, 0);
+#pragma unroll
+  for (int s = 1; s < 64 / kSeg; ++s) nmax = max(nmax, __builtin_amdgcn_readlane(nrows, s * kSeg));
 
   // F[r+1][c] + (vertical cost of row r) - Q(c): 2 with unit costs.
   int32_t Fpm = kCost ? (hc - hq) + (vr - Vpre[rmax - 1]) : 2;
@@ -1269,17 +1299,16 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
     if (k >= nmax) return;  // uniform
     const bool act = k < nrows;
     const int32_t r = rmax - 1 - k;
-    const int32_t labr = __builtin_amdgcn_update_dpp(0, mlab, DPP_ROW_NEWBCAST(k), 0xf, 0xf, false);
-    const int32_t Rf = __builtin_amdgcn_update_dpp(0, mrf, DPP_ROW_NEWBCAST(k), 0xf, 0xf, false);
-    const int32_t sl = __builtin_amdgcn_update_dpp(0, mslot, DPP_ROW_NEWBCAST(k), 0xf, 0xf, false);
+    const int32_t labr = row_meta<k>(mlab);
+    const int32_t Rf = row_meta<k>(mrf);
+    const int32_t sl = row_meta<k>(mslot);
     const bool tree_row = Rf == rmax;
     // Vertical costs of this row and of the next, and F[r][cmax] (unit
     // costs: 1, 1 and rmax - r).
     int32_t vcur = 1, vnext = 1, frc = rmax - r;
     if constexpr (kCost) {
-      const int32_t vpr = __builtin_amdgcn_update_dpp(0, mvp, DPP_ROW_NEWBCAST(k), 0xf, 0xf, false);
-      const int32_t vpn = __builtin_amdgcn_update_dpp(
-          0, mvp, DPP_ROW_NEWBCAST(k + 1 < kTinySeg ? k + 1 : k), 0xf, 0xf, false);
+      const int32_t vpr = row_meta<k>(mvp);
+      const int32_t vpn = row_meta<k + 1 < kSeg ? k + 1 : k>(mvp);
       vcur = vp_prev - vpr;
       vnext = vpr - vpn;
       frc = vr - vpr;
@@ -1287,8 +1316,7 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
     }
     // F[r+1][c+1] - Q(c+1): the left neighbour's previous F (lane t-1), or
     // the boundary F[r+1][cmax] for t = 0.
-    const int32_t fpr = __builtin_amdgcn_update_dpp(left_prev, Fpm - vcur - negt, DPP_ROW_SHR(1),
-                                                    0xf, 0xf, false);
+    const int32_t fpr = seg_shr1<kSeg>(Fpm - vcur - negt, left_prev, t);
     int32_t rename;
     if constexpr (kCost)
       rename = A.ren[(rows_x ? labr * A.alphabet + labc : labc * A.alphabet + labr) &
@@ -1300,7 +1328,7 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
     const int32_t far = (tree_row ? bfar : sv) + tdv[k] + negt;
     const int32_t dm = tree_row && tree_col ? diag : far;
     const int32_t carry = kCost ? frc + (hc - hq) : rmax - r + 1;
-    const int32_t h = min(carry, seg16_incl_min(min(dm, Fpm)));  // F - Q(c)
+    const int32_t h = min(carry, seg_incl_min<kSeg>(min(dm, Fpm)));  // F - Q(c)
     if (act) {
       Fpm = h + vnext;
       left_prev = frc;
@@ -1315,23 +1343,33 @@ __global__ __launch_bounds__(256) void tiny_kernel(DPArgs A, int32_t slot_ints) 
       }
     }
   };
-  static_for(row, std::make_integer_sequence<int, kTinySeg>{});
+  static_for(row, std::make_integer_sequence<int, kSeg>{});
 }
 
-void launch_tiny(const DPArgs& a, int slot_bytes, hipStream_t s) {
+template <int kSeg>
+void launch_tiny_seg(const DPArgs& a, int slot_bytes, hipStream_t s) {
+  constexpr int kPerGroup = 256 / kSeg;
   const int slot_ints = (slot_bytes + 3) / 4;  // per table, in 4-byte units
-  constexpr int64_t kChunk = kGridThreads / 256 * kTinyPerGroup;
+  constexpr int64_t kChunk = kGridThreads / 256 * kPerGroup;
   for (int64_t off = 0; off < a.ntasks; off += kChunk) {
     DPArgs c = a;
     c.tasks = a.tasks + off;
     c.descs = a.descs + off;
     c.ntasks = std::min(kChunk, a.ntasks - off);
-    const dim3 grid((unsigned)((c.ntasks + kTinyPerGroup - 1) / kTinyPerGroup));
-    const size_t lds = (size_t)kTinyPerGroup * slot_ints * 4;
+    const dim3 grid((unsigned)((c.ntasks + kPerGroup - 1) / kPerGroup));
+    const size_t lds = (size_t)kPerGroup * slot_ints * 4;
     if (a.costs)
-      hipLaunchKernelGGL(tiny_kernel<true>, grid, dim3(256), lds, s, c, slot_ints);
+      hipLaunchKernelGGL((tiny_kernel<kSeg, true>), grid, dim3(256), lds, s, c, slot_ints);
     else
-      hipLaunchKernelGGL(tiny_kernel<false>, grid, dim3(256), lds, s, c, slot_ints);
+      hipLaunchKernelGGL((tiny_kernel<kSeg, false>), grid, dim3(256), lds, s, c, slot_ints);
+  }
+}
+
+void launch_tiny(int form, const DPArgs& a, int slot_bytes, hipStream_t s) {
+  switch (form) {
+    case 0: launch_tiny_seg<16>(a, slot_bytes, s); break;
+    case 1: launch_tiny_seg<32>(a, slot_bytes, s); break;
+    default: launch_tiny_seg<64>(a, slot_bytes, s); break;
   }
 }
 
@@ -1480,8 +1518,8 @@ hipError_t forest_init_attributes(int limit) {
 // Launches one (class, task list). Returns false for a class without a kernel.
 bool launch_forest(int cls, const DPArgs& a, int lds_bytes, hipStream_t s) {
   if (a.ntasks <= 0) return true;
-  if (cls == kClassTiny) {
-    launch_tiny(a, lds_bytes, s);
+  if (is_tiny(cls)) {
+    launch_tiny(cls - kClassTiny, a, lds_bytes, s);
     return true;
   }
   if (cls >= 1 && cls <= kNumRowClasses) {

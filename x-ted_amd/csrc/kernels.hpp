@@ -188,7 +188,8 @@ hipError_t forest_init_attributes(int lds_limit);
 // launch_leaf, class kClassXL has none). Column-blocked classes launch
 // a.ntasks * a.xl_nbmax workgroups.
 bool launch_forest(int cls, const DPArgs& a, int lds_bytes, hipStream_t s);
-// Tiny tables (class kClassTiny), 16 per workgroup; slot_bytes per table.
-void launch_tiny(const DPArgs& a, int slot_bytes, hipStream_t s);
+// Tiny tables of form k (class kClassTiny + k), 256 / tiny_seg(k) per
+// workgroup; slot_bytes per table.
+void launch_tiny(int form, const DPArgs& a, int slot_bytes, hipStream_t s);
 
 }  // namespace xted

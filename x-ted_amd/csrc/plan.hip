@@ -61,22 +61,23 @@ __device__ __forceinline__ void classify(const PlanArgs& A, const TreeMeta& tx,
       need = n;
     }
   }
-  // A forced row-scan form (or the tiny form) takes every on-chip table it
-  // can hold.
-  if (A.force_class == kClassTiny && cls >= 1 && (cls <= kNumRowClasses || cls == kClassTiny)) {
+  // A forced row-scan form (or tiny form) takes every on-chip table it can
+  // hold.
+  if (is_tiny(A.force_class) && cls >= 1 && (cls <= kNumRowClasses || is_tiny(cls))) {
+    const int k = A.force_class - kClassTiny;
     const int cols = a <= b ? b : a;
     const int64_t row_need = (int64_t)slots * (leaves_c + 1) * 2;
-    if (cols <= kTinyCols && row_need <= kTinyMaxLds) {
-      cls = kClassTiny;
+    if (cols <= tiny_seg(k) && row_need <= tiny_max_lds(k)) {
+      cls = A.force_class;
       need = row_need;
     }
   }
   if (A.force_class >= 1 && A.force_class <= kNumRowClasses && cls >= 1 &&
-      (cls <= kNumRowClasses || cls == kClassTiny)) {
+      (cls <= kNumRowClasses || is_tiny(cls))) {
     const int wi = (A.force_class - 1) / 5, ci = (A.force_class - 1) % 5;
     const int cols = a <= b ? b : a;
     const int64_t row_need = (int64_t)slots * (leaves_c + 1) * 2;
-    if ((cls < kClassXL || cls == kClassTiny) && cols <= 64 * (1 << wi) * (1 << ci) &&
+    if ((cls < kClassXL || is_tiny(cls)) && cols <= 64 * (1 << wi) * (1 << ci) &&
         row_need <= A.rule.lds_max) {
       cls = A.force_class;
       need = row_need;

@@ -40,7 +40,8 @@ def short(name):
             return f"xl_bw{64 * w * c}{cost}"
         return f"rowscan_w{w}_c{c}{cost}"
     if base.startswith("tiny_kernel<"):
-        return "tiny_kernel_cost" if base == "tiny_kernel<true>" else "tiny_kernel"
+        parts = [p.strip() for p in base[len("tiny_kernel<"):-1].split(",")]
+        return f"tiny_s{parts[0]}" + ("_cost" if parts[-1] == "true" else "")
     return base
 
 
@@ -141,7 +142,8 @@ def resources(src, hipcc="/opt/rocm/bin/hipcc", arch="gfx950"):
                      "plan_scan_kernel", "plan_clear_kernel", "gather_dist_kernel"):
             if name in mangled:
                 if name == "tiny_kernel":
-                    return "tiny_kernel_cost" if "ILb1E" in mangled else "tiny_kernel"
+                    m = re.search(r"tiny_kernelILi(\d+)ELb([01])E", mangled)
+                    return f"tiny_s{m.group(1)}" + ("_cost" if m.group(2) == "1" else "")
                 if name == "plan_kernel":
                     return "plan_kernel<true>" if "ILb1E" in mangled else "plan_kernel<false>"
                 return name

@@ -26,16 +26,17 @@ ERR_CANCELLED = 10
 
 CLASS_AUTO = -1
 CLASS_LANE = 0
-NUM_CLASSES = 34
-CLASS_TINY = 33
+NUM_CLASSES = 36
+CLASS_TINY = 33  # tiny forms 33, 34, 35: 16, 32, 64 lanes per table
+TINY_SEG = (16, 32, 64)
 XL_BW = (4096, 2048, 1024, 512, 256, 128)  # column-block widths of classes 27..32
 
 
 def class_name(cls):
     """'leaf' (closed form for tables with a leaf keyroot) or
     'rowscan_w{W}_c{C}' (W waves, C columns per lane) or 'xl_bw{B}' (column
-    blocks of B columns, one workgroup each) or 'tiny_s16' (tables of at most
-    16 columns, four to a wave) or 'too_large'."""
+    blocks of B columns, one workgroup each) or 'tiny_s{S}' (tables of at
+    most S = 16, 32 or 64 columns, 64 / S to a wave) or 'too_large'."""
     if cls == 0:
         return "leaf"
     if 1 <= cls <= 25:
@@ -43,8 +44,8 @@ def class_name(cls):
         return f"rowscan_w{1 << wi}_c{1 << ci}"
     if 27 <= cls < 27 + len(XL_BW):
         return f"xl_bw{XL_BW[cls - 27]}"
-    if cls == CLASS_TINY:
-        return "tiny_s16"
+    if CLASS_TINY <= cls < CLASS_TINY + len(TINY_SEG):
+        return f"tiny_s{TINY_SEG[cls - CLASS_TINY]}"
     return "too_large"
 
 GEN_RRT, GEN_DEEP, GEN_SHALLOW = 0, 1, 2
@@ -82,6 +83,7 @@ _SIGNATURES = {
     "xted_set_level_align": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_size_bands": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_set_light_lds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "xted_set_tiny_forms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "xted_bracket_to_lines": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64,
                                              ctypes.c_char_p, ctypes.c_int64,
                                              ctypes.POINTER(ctypes.c_int64),
@@ -381,6 +383,11 @@ class Context:
     def set_light_lds(self, bytes_per_wave):
         """Size-band mode 2: slot bytes per wave above which a table is heavy."""
         _check(lib().xted_set_light_lds(self._h, int(bytes_per_wave)))
+
+    def set_tiny_forms(self, mask=1):
+        """Bit k: tables of at most 16 << k columns take tiny form k (class
+        CLASS_TINY + k), one per (16 << k)-lane segment of a wave."""
+        _check(lib().xted_set_tiny_forms(self._h, int(mask)))
 
     def set_level_align(self, mode):
         """0: every pair starts at level 0; 1: every pair ends at the last
