@@ -341,6 +341,10 @@ void launch_leaf(const LeafArgs& a, int64_t ntiles, hipStream_t s) {
 typedef uint32_t u32x4a2 __attribute__((ext_vector_type(4), aligned(2)));
 typedef uint32_t u32x2a2 __attribute__((ext_vector_type(2), aligned(2)));
 typedef uint32_t u32x1a2 __attribute__((aligned(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+// Word 3 of a raw buffer resource on CDNA (32-bit data format, no swizzle).
+constexpr int32_t kBufferWord3 = 0x00020000;
 
 // ---------------------------------------------------------------------------
 // Column-blocked form (kXL). A table too wide or too LDS-hungry for one
@@ -726,26 +730,70 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // partly valid group reads up to E-1 entries before column c0 (the arena
   // keeps kTdPad entries of padding before and after every table, so both
   // stay inside the allocation).
-  auto load_row = [&](int32_t rr, uint32_t* out) {
-    const td_t* row = TDo + (int64_t)rr * ldo;
+  // On-chip forms read through a buffer resource over the table (its
+  // kTdPad entries of padding included): the row offset is a scalar and each
+  // lane's column offset a VGPR fixed for the whole table, so a row's loads
+  // write no VGPR but their destinations (with 64-bit per-lane addresses
+  // the compiler formed each row's address in the ring registers, which made
+  // the loop head wait for every load in flight), and the 32-bit offsets
+  // free registers (rowscan_w2_c8 80 VGPRs against 90; the 4096-pair batch
+  // 99.4 against 101.3 ms). The column-blocked forms, whose rows also wait
+  // on carry and import words, ran slower that way (the 20,000-node pair
+  // 227 against 217 ms) and keep 64-bit addresses.
+  const __amdgpu_buffer_rsrc_t tdrs =
+      __builtin_amdgcn_make_buffer_rsrc((void*)(TDo - kTdPad), 0, -1, kBufferWord3);
+  // Column offset of each group: in entries from the table (kXL), else in
+  // bytes from the resource base.
+  int32_t roff[G];
 #pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const int32_t tb = TCOL(g, 0);
-      const int32_t off = tb < ncols ? cmax - E - tb : c0;
-      if constexpr (E == 8) {
-        const u32x4a2 v = *reinterpret_cast<const u32x4a2*>(row + off);
-        out[g * 4 + 0] = v.x;
-        out[g * 4 + 1] = v.y;
-        out[g * 4 + 2] = v.z;
-        out[g * 4 + 3] = v.w;
-      } else if constexpr (E == 4) {
-        const u32x2a2 v = *reinterpret_cast<const u32x2a2*>(row + off);
-        out[g * 2 + 0] = v.x;
-        out[g * 2 + 1] = v.y;
-      } else if constexpr (E == 2) {
-        out[g] = *reinterpret_cast<const u32x1a2*>(row + off);
-      } else {
-        out[g] = row[off];
+  for (int g = 0; g < G; ++g) {
+    const int32_t tb = TCOL(g, 0);
+    const int32_t off = tb < ncols ? cmax - E - tb : c0;
+    roff[g] = kXL ? off : (off + kTdPad) * (int32_t)sizeof(td_t);
+  }
+  auto load_row = [&](int32_t rr, uint32_t* out) {
+    if constexpr (kXL) {
+      const td_t* row = TDo + (int64_t)rr * ldo;
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const td_t* p = row + roff[g];
+        if constexpr (E == 8) {
+          const u32x4a2 v = *reinterpret_cast<const u32x4a2*>(p);
+          out[g * 4 + 0] = v.x;
+          out[g * 4 + 1] = v.y;
+          out[g * 4 + 2] = v.z;
+          out[g * 4 + 3] = v.w;
+        } else if constexpr (E == 4) {
+          const u32x2a2 v = *reinterpret_cast<const u32x2a2*>(p);
+          out[g * 2 + 0] = v.x;
+          out[g * 2 + 1] = v.y;
+        } else if constexpr (E == 2) {
+          out[g] = *reinterpret_cast<const u32x1a2*>(p);
+        } else {
+          out[g] = *p;
+        }
+      }
+    } else {
+      // rr * ldo < 2^30 entries (trees of at most 32767 nodes): an unsigned
+      // 32-bit byte offset.
+      const int32_t so = (int32_t)((uint32_t)rr * (uint32_t)ldo * (uint32_t)sizeof(td_t));
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        if constexpr (E == 8) {
+          const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(tdrs, roff[g], so, 0);
+          out[g * 4 + 0] = v.x;
+          out[g * 4 + 1] = v.y;
+          out[g * 4 + 2] = v.z;
+          out[g * 4 + 3] = v.w;
+        } else if constexpr (E == 4) {
+          const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(tdrs, roff[g], so, 0);
+          out[g * 2 + 0] = v.x;
+          out[g * 2 + 1] = v.y;
+        } else if constexpr (E == 2) {
+          out[g] = __builtin_amdgcn_raw_buffer_load_b32(tdrs, roff[g], so, 0);
+        } else {
+          out[g] = __builtin_amdgcn_raw_buffer_load_b16(tdrs, roff[g], so, 0);
+        }
       }
     }
   };
