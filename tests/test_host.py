@@ -169,3 +169,30 @@ def test_command_line_bracket_conversion(tmp_path):
                     str(out_n), str(out_a)], check=True, timeout=60)
     assert out_n.read_text() == "\n".join(nodes)
     assert out_a.read_text() == "\n".join(adjs)
+
+
+def test_autotune_accepts_only_non_overlapping_wins(monkeypatch):
+    """The tuner's acceptance rule on scripted step times (no device): a
+    candidate whose steps overlap the best rule's fastest step is logged but
+    not taken; one whose every step is faster becomes the best, and later
+    candidates are compared with it."""
+    import autotune
+    times = {8: [1.00, 1.08, 1.12, 1.02, 1.05, 1.10],  # the default rule (6 steps)
+             16: [0.95, 0.97, 1.01],                     # overlaps 1.00: rejected
+             4: [0.90, 0.92, 0.93],                      # every step < 1.00: new best
+             2: [0.89, 0.91, 0.95]}                      # overlaps 0.90: rejected
+    dist = np.arange(5, dtype=np.int32)
+
+    def fake_steps(ctx, xs, ys, rule, reps):
+        return list(times[rule["max_cpl"]]), dist
+
+    monkeypatch.setattr(autotune, "steps", fake_steps)
+    monkeypatch.setattr(autotune, "apply", lambda ctx, rule: None)
+    monkeypatch.setattr(autotune, "FINE_SPAN", 0)
+    logged = []
+    best, results, spread = autotune.tune(None, None, None, stages=[{"max_cpl": [16, 4, 2]}],
+                                          log=logged.append)
+    assert best["max_cpl"] == 4
+    assert [r["max_cpl"] for r, _ in results] == [8, 16, 4, 2]
+    assert spread == pytest.approx(0.12)
+    assert sum("new best" in m for m in logged) == 1

@@ -21,10 +21,10 @@ on:
 
 The workload is the 4096-pair uneven batch's generator on a seed stream of
 its own (not the one bench.py times). The rule starts as the default; a
-candidate replaces the current best only when it is faster by more than
-the spread measured over repeated steps of the default rule, so noise
-cannot pick a rule. Every candidate must reproduce the default rule's
-distances exactly, or the run stops.
+candidate replaces the current best only when every one of its steps is
+faster than the best rule's fastest step, so step-to-step noise cannot pick
+a rule. Every candidate must reproduce the default rule's distances
+exactly, or the run stops.
 
     python x-ted_amd/autotune.py [--pairs N] [--out x-ted_amd/tuning.json]
 """
@@ -105,6 +105,9 @@ def tune(ctx, xs, ys, stages=STAGES, reps=3, log=print):
     base_t = min(base_times)
     results = [(dict(DEFAULT), base_t)]
     log(f"default {DEFAULT}: {base_t * 1e3:.1f} ms, step-to-step spread {spread * 1e3:.2f} ms")
+    # The best rule's fastest step: a candidate wins only when its slowest
+    # step beats it (every step of the candidate faster than every step of
+    # the best), so neither rule's step-to-step noise can pick it.
     best, best_t = dict(DEFAULT), base_t
 
     def trial(rule):
@@ -120,8 +123,9 @@ def tune(ctx, xs, ys, stages=STAGES, reps=3, log=print):
             raise RuntimeError(f"rule {rule} changed a distance")
         t = min(times)
         results.append((rule, t))
-        won = t < best_t - spread
-        log(f"{rule}: {t * 1e3:.1f} ms{' (new best)' if won else ''}")
+        won = max(times) < best_t
+        log(f"{rule}: {t * 1e3:.1f} ms (slowest step {max(times) * 1e3:.1f})"
+            f"{' (new best)' if won else ''}")
         if won:
             best, best_t = rule, t
 
