@@ -285,18 +285,29 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(PlanArgs A) {
   const bool leaf_form = A.pair_leaf ? A.pair_leaf[p] != 0 : A.lane_enable != 0;
   ht_clear(T);
   __syncthreads();
+  // Table idx of the chunk is (l, k) = (idx / Kx, idx % Kx); a chunk holds
+  // at most max(lchunk_tables, Kx) < 2^15 tables. One division per thread,
+  // then each step of kPlanThreads tables adds (dl, dk) with one carry.
   auto sweep = [&](auto body) {
-    for (int64_t base = 0; base < ch.total; base += kPlanThreads) {
-      const int64_t idx = base + threadIdx.x;
-      const bool active = idx < ch.total;
+    const int32_t total = (int32_t)ch.total;
+    const int32_t kx = max(ch.Kx, 1);
+    const int32_t dl = kPlanThreads / kx, dk = kPlanThreads % kx;
+    int32_t lq = (int32_t)threadIdx.x / kx, kr = (int32_t)threadIdx.x % kx;
+    for (int32_t base = 0; base < total; base += kPlanThreads) {
+      const bool active = base + (int32_t)threadIdx.x < total;
       int bucket = 0, lds = 0;
       uint32_t i0 = 0, j0 = 0;
       unsigned long long cells = 0;
-      if (active) {
-        const int l = ch.lbase + ch.l_lo + (int)(idx / ch.Kx), k = ch.kbase + (int)(idx % ch.Kx);
-        classify(A, tx, ty, loff, leaf_form, k, l, &bucket, &lds, &i0, &j0, &cells);
-      }
+      if (active)
+        classify(A, tx, ty, loff, leaf_form, ch.kbase + kr, ch.lbase + ch.l_lo + lq, &bucket,
+                 &lds, &i0, &j0, &cells);
       body(active, bucket, lds, i0, j0, cells);
+      lq += dl;
+      kr += dk;
+      if (kr >= kx) {
+        kr -= kx;
+        ++lq;
+      }
     }
   };
   if (!kScatter) {
