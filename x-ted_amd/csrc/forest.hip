@@ -815,12 +815,15 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // (label, Rf = orl_R(r)+1, slots = fslot | wslot << 16 | wrow << 31) and,
   // for kXL, the saved row's index (leaf rank of r-1) or -1; read per row
   // with readlane; the next chunk is loaded one chunk ahead.
+  auto meta_rec = [&](int q) {
+    return rrecR[max(rmax - 1 - (64 * q + lane), r0)];  // rows past the table read row r0
+  };
   auto load_meta = [&](int q, int32_t& mlab, int32_t& mrf, int32_t& mslot, int32_t& mrid,
-                       int32_t& mvp) {
+                       int32_t& mvp, const NodeRec* pre = nullptr) {
     const int32_t rr = rmax - 1 - (64 * q + lane);
     const int32_t r = max(rr, r0);  // rows past the table read row r0
     const int32_t rp = max(r - 1, r0);
-    const NodeRec rc = rrecR[r];  // orl(r), label, kdep(r), kdep(r-1) | r-1 a leaf
+    const NodeRec rc = pre ? *pre : rrecR[r];  // orl(r), label, kdep(r), kdep(r-1) | r-1 a leaf
     const int32_t orr = rc.orl, lab = rc.label, kd = rc.x;
     const int32_t kdp = rc.y & 0x7FFFFFFF;
     const int32_t fs = (orr + 1 < rmax) ? kd - base - 1 : 0;
@@ -838,8 +841,16 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   const int32_t nchunks = (nrows + 63) >> 6;
   int32_t mlab, mrf, mslot, mrid = -1, nlab = 0, nrf = 0, nslot = 0, nrid = -1;
   int32_t mvp = 0, nvp = 0;  // kCost: Vpre of the row
-  load_meta(0, mlab, mrf, mslot, mrid, mvp);
-  if (nchunks > 1) load_meta(1, nlab, nrf, nslot, nrid, nvp);
+  {
+    // The first two chunks' records are read before either is used (one
+    // load latency in the set-up, not two), each as one whole record: with
+    // kdep pinned the compiler does not load it in a branch of its own after
+    // a wait for the rest.
+    NodeRec rc0 = meta_rec(0), rc1 = meta_rec(1);
+    asm volatile("" : "+v"(rc0.x), "+v"(rc1.x));
+    load_meta(0, mlab, mrf, mslot, mrid, mvp, &rc0);
+    if (nchunks > 1) load_meta(1, nlab, nrf, nslot, nrid, nvp, &rc1);
+  }
 
   // Column-blocked form: the upstream carries are read a window (64 rows) at
   // a time, one window ahead, lane i holding row 64j + i's word (cwl / cwh,
@@ -970,11 +981,20 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
 #pragma unroll
     for (int u = 0; u < S; ++u) {
       const int32_t k = k0 + u;  // rows done so far
-      if (k >= nrows) break;
+      // With a ring of at most 4 slots every slot issues its prefetch, also
+      // past the last row, so that every path into the loop head has issued
+      // the same loads and no wait there drains the ring (longer rings keep
+      // the early exit: short tables would step through up to 15 idle slots).
+      if constexpr (S <= 4) {
+        load_row(max(rmax - 1 - k - PD, r0), tdq[(u + PD) % S]);
+        if (k >= nrows) continue;
+      } else {
+        if (k >= nrows) break;
+        load_row(max(rmax - 1 - k - PD, r0), tdq[(u + PD) % S]);
+      }
       const int32_t i = k & 63;  // index in the metadata chunk
       const int32_t r = rmax - 1 - k;
       const int32_t ku = kCost ? 0 : k;  // the unit form's row shift
-      load_row(max(r - PD, r0), tdq[(u + PD) % S]);
       if constexpr (kXL) {
         constexpr int32_t kM = kXLEvery - 1, kHalf = kXLEvery / 2;
         if ((k & kM) == 0 && k > 0) {
