@@ -166,7 +166,8 @@ int xted_all_pairs(xted_context* ctx, int32_t ntrees, const int32_t* sizes,
  * Granularity cut-overs, the role of the fixed thresholds 15 / 18,750 /
  * 25,250,000 in X-TED_GPU/parallel_X_TED.cu:382. lds_per_wave: slot bytes
  * one wave may own before a table gets more waves; max_cpl (1, 2, 4, 8 or
- * 16): columns per lane before a table gets more waves; tables with at
+ * 16): columns per lane before a table gets more waves (at most 8 under a
+ * cost model, whose 16-column forms run at 2 waves per SIMD); tables with at
  * least wide_cols columns get at least min_waves waves. */
 int xted_set_tuning(xted_context* ctx, int32_t lds_per_wave, int32_t max_cpl,
                     int32_t min_waves, int32_t wide_cols);

@@ -200,11 +200,14 @@ constexpr int kMaxCols = 64 * 16 * kMaxRowWaves;  // 16384
 // 16 columns per lane at 16 waves would spill); wider tables are column-blocked.
 constexpr int kMaxOnChipCols = 64 * 8 * kMaxRowWaves;  // 8192
 
-// 16-lane form only: the 32-lane form ran the same 4096-pair batch in the
-// same time (100.6 against 100.9 ms unit, 125.0 against 124.4 ms under a
-// cost model) and the 64-lane one 4% slower (4 waves per SIMD for its 64
-// TD registers per lane).
+// 16-lane form only by default. Under the default rule the 32-lane form ran
+// the 4096-pair batch in the same time (100.6 against 100.9 ms unit, 125.0
+// against 124.4 ms under a cost model); the tuned rule (x-ted_amd/tuning.json:
+// up to 16 columns per lane) turns it on (96.5 against 99.6 ms). The 64-lane
+// form ran 4% slower (4 waves per SIMD for its 64 TD registers per lane).
 constexpr int32_t kTinyDefault = 1;
+// Most columns per lane a table takes under a cost model.
+constexpr int32_t kMaxCostCpl = 8;
 
 // Tuning knobs carried by the plan builder (autotuner writes these).
 struct ClassRule {

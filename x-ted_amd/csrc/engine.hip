@@ -593,6 +593,10 @@ int Context::plan(Group& g, int32_t npairs, const int32_t* xs, const int32_t* ys
   pa.nlevels = nlevels;
   pa.lchunk_tables = plan_chunk;
   pa.rule = rule;
+  // The cost forms with 16 columns per lane hold 169-176 VGPRs (2 waves per
+  // SIMD): under a cost model a table gets more waves instead (the 4096-pair
+  // batch under a cost model ran 125.5 ms against 131.3 with up to 16).
+  if (cost_alphabet_ > 0) pa.rule.max_cpl = std::min(pa.rule.max_cpl, kMaxCostCpl);
   pa.rule_tail = ClassRule{lds_limit, 1, 1 << 30, lds_limit, 16, rule.xl_seq_rows};
   pa.rule_tail.xl_first = rule.xl_first;
   pa.rule_tail.tiny = rule.tiny;
