@@ -497,10 +497,12 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   constexpr int E = C <= 8 ? C : 8;  // contiguous columns per group
   constexpr int G = C / E;           // groups per lane
   // TD ring slots (a power of two dividing 64): up to 32 registers of ring,
-  // so narrow lanes prefetch more rows ahead.
+  // so narrow lanes prefetch more rows ahead; 16 columns per lane take 4
+  // slots as well (with 2, one row ahead, `rowscan_w1_c16` ran 2% slower at
+  // 127 VGPRs against 141 with 4).
   // (The column-blocked form also rings its imports, and the cost form's
   // longer row body has to unroll S times: at most 8 slots for both.)
-  constexpr int S0 = C >= 16 ? 2 : C >= 8 ? 4 : C >= 4 ? 8 : 16;
+  constexpr int S0 = C >= 8 ? 4 : C >= 4 ? 8 : 16;
   constexpr int S = kXL && S0 > 4 ? 4 : kCost && S0 > 8 ? 8 : S0;
   constexpr int PD = S - 1;  // rows of TD prefetched
   constexpr int BW = 64 * W * C;                    // columns of one block
