@@ -969,8 +969,9 @@ __global__ __launch_bounds__(64 * W) void rowscan_kernel(DPArgs A) {
   // so row k of the table always uses ring slot k % S and every slot index is
   // static: row k reads slot u and prefetches row k+PD into slot (u+PD) % S,
   // the slot the previous row has just finished with. Only the last
-  // iteration leaves the unrolled rows early, and it leaves the loop, so the
-  // prefetches in flight never have to drain at the loop head.
+  // iteration has rows past the table's end: rings of at most 4 slots still
+  // issue those rows' prefetches and skip their bodies, longer rings leave the
+  // unrolled rows early (see the loop body).
   for (int32_t k0 = 0; k0 < nrows; k0 += S) {
     if ((k0 & 63) == 0 && k0 > 0) {  // next 64-row chunk of row metadata
       mlab = nlab;
