@@ -191,8 +191,61 @@ def test_autotune_accepts_only_non_overlapping_wins(monkeypatch):
     monkeypatch.setattr(autotune, "FINE_SPAN", 0)
     logged = []
     best, results, spread = autotune.tune(None, None, None, stages=[{"max_cpl": [16, 4, 2]}],
-                                          log=logged.append)
+                                          log=logged.append, confirm_top=1)
     assert best["max_cpl"] == 4
     assert [r["max_cpl"] for r, _ in results] == [8, 16, 4, 2]
     assert spread == pytest.approx(0.12)
     assert sum("new best" in m for m in logged) == 1
+
+
+def test_autotune_confirmation_takes_lowest_median(monkeypatch):
+    """After the search, the best rule and the next fastest candidates are
+    re-timed in interleaved rounds and the lowest median step wins: here the
+    search's pick (one fast step, slow others) loses to a steadier rule whose
+    best step did not beat it."""
+    import autotune
+    times = {8: [1.00, 1.01, 1.02, 1.00, 1.01, 1.02],
+             4: [0.80, 0.99, 0.99],   # every step < 1.00: the search's pick
+             2: [0.90, 0.91, 0.92]}   # overlaps 0.80 in the search; median 0.91
+    dist = np.zeros(3, dtype=np.int32)
+    calls = []
+
+    def fake_steps(ctx, xs, ys, rule, reps):
+        calls.append(rule["max_cpl"])
+        return list(times[rule["max_cpl"]]), dist
+
+    monkeypatch.setattr(autotune, "steps", fake_steps)
+    monkeypatch.setattr(autotune, "apply", lambda ctx, rule: None)
+    monkeypatch.setattr(autotune, "FINE_SPAN", 0)
+    confirmed = []
+    best, results, _ = autotune.tune(None, None, None, stages=[{"max_cpl": [4, 2]}],
+                                     log=lambda m: None, confirm_top=3, confirmed=confirmed)
+    assert best["max_cpl"] == 2
+    # finalists: the search's pick first, then by best step (2, then the default)
+    assert [r["max_cpl"] for r, _ in confirmed] == [4, 2, 8]
+    assert calls[3:] == [4, 2, 8] * autotune.CONFIRM_ROUNDS
+
+
+def test_autotune_confirmation_averages_workloads(monkeypatch):
+    """With a second workload, the confirmation scores each finalist by the
+    mean of its median steps on both: a rule that is fast on the tuning
+    workload but slow on the second one loses."""
+    import autotune
+    times = {8: [1.00, 1.01, 1.02, 1.00, 1.01, 1.02], 4: [0.80, 0.99, 0.99], 2: [0.90, 0.91, 0.92]}
+    second = {8: [1.0] * 3, 4: [0.9] * 3, 2: [1.2] * 3}
+    dist = np.zeros(3, dtype=np.int32)
+
+    def fake_steps(ctx, xs, ys, rule, reps):
+        table = second if xs == "x2" else times
+        return list(table[rule["max_cpl"]]), dist
+
+    monkeypatch.setattr(autotune, "steps", fake_steps)
+    monkeypatch.setattr(autotune, "apply", lambda ctx, rule: None)
+    monkeypatch.setattr(autotune, "FINE_SPAN", 0)
+    confirmed = []
+    best, _, _ = autotune.tune(None, None, None, stages=[{"max_cpl": [4, 2]}], log=lambda m: None,
+                               confirm_top=3, confirmed=confirmed, confirm_sets=[("x2", "y2")])
+    assert best["max_cpl"] == 4
+    scores = {r["max_cpl"]: sc for r, sc in confirmed}
+    assert scores[4] == pytest.approx((0.99 + 0.9) / 2)
+    assert scores[2] == pytest.approx((0.91 + 1.2) / 2)

@@ -14,10 +14,15 @@ on:
   1. the schedule: level alignment x size bands (none, by rows, by rows and
      light / heavy slot storage);
   2. the tail rule;
-  3. a coarse grid of size bands x slot bytes per wave x columns per lane;
+  3. a coarse grid of size bands x slot bytes per wave x columns per lane x
+     tiny forms (16 lanes; 16 and 32);
   4. the slot bytes per wave above which a table is heavy (size bands 2);
   5. the tiny forms (tables of up to 16, 32 or 64 columns);
-  6. a fine grid of slot bytes per wave (1 KiB steps) around the best.
+  6. a fine grid of slot bytes per wave (1 KiB steps) around the best;
+  7. a confirmation: the best rule and the three next fastest candidates
+     re-timed in two interleaved rounds on the tuning workload and on a
+     second one (seed stream 1005), the lowest mean of the two median steps
+     winning.
 
 The workload is the 4096-pair uneven batch's generator on a seed stream of
 its own (not the one bench.py times). The rule starts as the default; a
@@ -46,13 +51,16 @@ DEFAULT = {"lds_per_wave": 10 * 1024, "max_cpl": 8, "min_waves": 1, "wide_cols":
            "tiny_forms": 1}
 # Seed stream of the tuning workload (bench.py times stream 3).
 SEED_TUNE = 1003
+SEED_CONFIRM = 1005  # second workload of the confirmation round
 COARSE_LDS = [6 * 1024, 10 * 1024, 14 * 1024, 20 * 1024, 28 * 1024]
 STAGES = [
     {"level_align": [0, 1, 2], "size_bands": [0, 1, 2]},
     {"tail_pairs": [0, 8, 32]},
     # Slot bytes per wave set which tables share a launch, and with it how
     # far splitting light from heavy tables raises occupancy: searched together.
-    {"size_bands": [1, 2], "lds_per_wave": COARSE_LDS, "max_cpl": [8, 16]},
+    # The 32-lane tiny form pays only with some of these rules, so it is
+    # searched with them too.
+    {"size_bands": [1, 2], "lds_per_wave": COARSE_LDS, "max_cpl": [8, 16], "tiny_forms": [1, 3]},
     # Where heavy tables start (size-band mode 2 only).
     {"light_lds": [6 * 1024, 8 * 1024, 10 * 1024, 12 * 1024, 14 * 1024]},
     # Which tables the tiny forms take (16-lane only, up to 32 or 64 columns).
@@ -60,6 +68,10 @@ STAGES = [
 ]
 FINE_STEP = 1024
 FINE_SPAN = 3  # steps either side of the coarse best
+# Confirmation: the best rule and the next fastest candidates re-timed in
+# interleaved rounds (their best single steps are within noise of each other).
+CONFIRM_TOP = 4
+CONFIRM_ROUNDS = 2
 TUNING_FILE = os.path.join(HERE, "tuning.json")
 
 
@@ -97,9 +109,13 @@ def steps(ctx, xs, ys, rule, reps):
     return times, dist
 
 
-def tune(ctx, xs, ys, stages=STAGES, reps=3, log=print):
+def tune(ctx, xs, ys, stages=STAGES, reps=3, log=print, confirm_top=CONFIRM_TOP, confirmed=None,
+         confirm_sets=()):
     """Returns (best_rule, results, spread): the staged search described in
-    the module docstring. results lists (rule, best step time)."""
+    the module docstring. results lists (rule, best step time); confirmed, if
+    given, receives (rule, score) of the confirmation round, the score being
+    the mean over (xs, ys) and the workloads of confirm_sets (pairs of id
+    arrays) of each one's median step time."""
     base_times, base_d = steps(ctx, xs, ys, DEFAULT, 6)
     spread = max(base_times) - min(base_times)
     base_t = min(base_times)
@@ -140,6 +156,29 @@ def tune(ctx, xs, ys, stages=STAGES, reps=3, log=print):
         lds = start["lds_per_wave"] + k * FINE_STEP
         if lds >= 2 * 1024:
             trial(dict(start, lds_per_wave=lds))
+    # Confirmation: the best rule and the fastest other candidates (by their
+    # best step), re-timed in interleaved rounds; the lowest median step wins.
+    ranked = [r for r, _ in sorted(results, key=lambda rt: rt[1]) if r != best]
+    finalists = [best] + ranked[:max(0, confirm_top - 1)]
+    sets = [(xs, ys, base_d)]
+    for cx, cy in confirm_sets:
+        sets.append((cx, cy, steps(ctx, cx, cy, DEFAULT, 1)[1]))
+    samples = [[[] for _ in sets] for _ in finalists]
+    if len(finalists) > 1:
+        for _ in range(CONFIRM_ROUNDS):
+            for i, rule in enumerate(finalists):
+                for w, (sx, sy, sd) in enumerate(sets):
+                    times, d = steps(ctx, sx, sy, rule, reps)
+                    if not np.array_equal(d, sd):
+                        raise RuntimeError(f"rule {rule} changed a distance")
+                    samples[i][w].extend(times)
+        scores = [float(np.mean([np.median(t) for t in per])) for per in samples]
+        for rule, per, sc in zip(finalists, samples, scores):
+            meds = ", ".join(f"{np.median(t) * 1e3:.1f}" for t in per)
+            log(f"confirm {rule}: medians {meds} ms, mean {sc * 1e3:.1f} ms")
+            if confirmed is not None:
+                confirmed.append((rule, sc))
+        best = finalists[int(np.argmin(scores))]
     apply(ctx, best)
     return best, results, spread
 
@@ -152,14 +191,21 @@ def main():
     sys.path.insert(0, os.path.dirname(HERE))
     from bench import batch_trees
     trees = batch_trees(args.pairs, cfg=SEED_TUNE)
+    trees2 = batch_trees(args.pairs, cfg=SEED_CONFIRM)
+    confirmed = []
     with xted.Context(0) as ctx:
         ids = ctx.pool_add(trees)
-        best, results, spread = tune(ctx, ids[0::2].copy(), ids[1::2].copy())
+        ids2 = ctx.pool_add(trees2)
+        best, results, spread = tune(ctx, ids[0::2].copy(), ids[1::2].copy(),
+                                     confirmed=confirmed,
+                                     confirm_sets=[(ids2[0::2].copy(), ids2[1::2].copy())])
     out = {"rule": best,
            "workload": f"batch{args.pairs}_uneven_n100-1000_deep50_shallow50_alphabet200 "
                        f"(seed stream {SEED_TUNE})",
            "spread_ms": round(spread * 1e3, 2),
-           "results_ms": [{"rule": r, "ms": round(t * 1e3, 2)} for r, t in results]}
+           "results_ms": [{"rule": r, "ms": round(t * 1e3, 2)} for r, t in results],
+           "confirm_workload": f"seed stream {SEED_CONFIRM}",
+           "confirm_mean_median_ms": [{"rule": r, "ms": round(t * 1e3, 2)} for r, t in confirmed]}
     with open(args.out, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps({"best": best, "spread_ms": round(spread * 1e3, 2)}))
